@@ -1,0 +1,181 @@
+"""Benchmark of the screening work unit on one MI355X: 20,000 synthetic stars x 4,087 synthetic clusters with
+the fused shuffled-velocity control (stars seed 2, clusters seed 0, control permutation seed 3), rows with
+p_r50 >= 0.001.  A step is one device pass over the whole unit (draws, tracks, scan, finish, row compaction)
+with the cluster tables resident and the star records already in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fast|plain] [--stars N] [--no-cpu]
+
+Prints one JSON line.  value = pairs screened per second (real + control pairs) of the chosen form;
+the plain FP64 form is timed beside it (plain_value, speedup) and their p_r50 agreement is checked.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from phygas_amd import hip  # noqa: E402
+from phygas_amd import workload as wl  # noqa: E402
+
+PEAK_FP64_VALU = 78.6     # TFLOP/s, MI355X vector FP64
+PEAK_FP32_VALU = 157.3    # TFLOP/s, MI355X vector FP32
+METRIC = "star-cluster pairs/s, chosen vs plain GPU at |dp_r50|<=1e-4; p50/p99 latency"
+
+
+def scan_flops(cn: np.ndarray, n_stars: int) -> float:
+    """Algorithmic FP64 work of the full scan: 126 FLOP per interior pair-point, 56 at a track's last point
+    (add/sub/mul/div/sqrt = 1 FLOP), summed over every cluster, times the stars."""
+    per_star = float(np.sum(126.0 * (cn - 1) + 56.0))
+    return per_star * n_stars
+
+
+def cpu_baseline(clusters, stars, threads: int) -> dict | None:
+    """The reference's own screening loop (oracle/_ref, built from the reference sources) or, without it, the
+    oracle restatement, on a bounded sample: about 50 stars per thread against every cluster."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "cpu_bench_ref")
+    port = os.path.join(ROOT, "oracle", "cpu_bench_orc")
+    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    if not os.path.exists(exe):
+        return None
+    n = 50 * threads
+    with tempfile.TemporaryDirectory() as d:
+        cf, sf = os.path.join(d, "clusters.txt"), os.path.join(d, "stars.txt")
+        wl.write_clusters(cf, clusters)
+        wl.write_targets(sf, stars[:n])
+        out = subprocess.run([exe, cf, sf, str(threads), str(n)], capture_output=True, text=True, check=True).stdout
+    kv = dict(x.split("=") for x in out.split())
+    return {"value": float(kv["pairs_per_s"]), "unit": "pairs/s", "cores": threads, "kind": kind,
+            "sample": f"{n} of the work unit's stars x {kv['clusters']} clusters, chi2 cut 25 + finish "
+                      f"({kv['kept']} kept pairs); cluster tracks prepared first in {float(kv['cluster_prep_s']):.2f} s "
+                      f"and not timed"}
+
+
+def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float, p_min: float):
+    for _ in range(warmup):
+        e.screen_async(form, chi2_max, 64, p_min)
+    e.sync()
+    e.set_timing(True)
+    stage_sum: dict[str, float] = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.screen_async(form, chi2_max, 64, p_min)
+        for k, v in e.stage_times().items():     # waits for the pass; events are on the engine's stream
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    e.sync()
+    t1 = time.perf_counter()
+    e.set_timing(False)
+    ms = (t1 - t0) * 1000.0 / steps
+    return ms, {k: v / steps for k, v in stage_sum.items()}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--form", choices=["fast", "plain"], default="fast")
+    ap.add_argument("--stars", type=int, default=20000)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the screening shards by star with no "
+              f"collective, see the sharding paragraph of the design notes", file=sys.stderr)
+        return 2
+
+    clusters = wl.make_clusters(4087, 0)
+    stars = wl.make_stars(args.stars, 2)
+    control = wl.shuffled_control(stars, 3)
+    chi2_max, p_min = 25.0, 0.001
+    cn = (np.ceil(clusters["tmax_myr"] / 0.5) + 1).astype(np.int64)
+    n_stars = 2 * len(stars)
+    pairs = n_stars * len(clusters)
+
+    e = hip.Engine(0)
+    t0 = time.perf_counter()
+    e.load_clusters(clusters, 64, 2.0)
+    tables_s = time.perf_counter() - t0
+    e.load_stars(stars, 0)
+    e.load_stars(control, 1)
+
+    form = hip.FAST if args.form == "fast" else hip.PLAIN
+    try:
+        ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
+    except hip.PhygasError as err:
+        if form == hip.FAST and "not built yet" in str(err):
+            print("fast form not available yet; timing the plain form", file=sys.stderr)
+            form = hip.PLAIN
+            ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
+        else:
+            raise
+    rows = e.rows()
+    kept = e.kept()
+    result = {"form": "fast" if form == hip.FAST else "plain", "ms": ms, "stages": stages, "rows": len(rows),
+              "kept": kept}
+
+    plain = None
+    if form == hip.FAST and not args.no_plain:
+        pms, pstages = run_form(e, hip.PLAIN, args.plain_steps, 0, chi2_max, p_min)
+        prow = e.rows()
+        a = {(r["star"], r["cluster"]): r["p_r50"] for r in rows}
+        b = {(r["star"], r["cluster"]): r["p_r50"] for r in prow}
+        common = set(a) & set(b)
+        dp = max((abs(a[k] - b[k]) for k in common), default=0.0)
+        nbad = sum(abs(a[k] - b[k]) > 1e-4 for k in common)
+        plain = {"ms": pms, "stages": pstages, "rows": len(prow), "value": pairs / (pms / 1000.0),
+                 "rows_only_fast": len(set(a) - set(b)), "rows_only_plain": len(set(b) - set(a)),
+                 "max_abs_dp_r50": dp, "pairs_dp_r50_above_1e-4": nbad}
+
+    # roofline of the dominant kernel: the scan of the plain form (FP64 VALU); for the fast form, its scan
+    if form == hip.PLAIN:
+        k_ms = stages.get("scan", float("nan"))
+        flops = scan_flops(cn, n_stars)
+        achieved = flops / (k_ms / 1000.0) / 1e12
+        roof = {"bound": "valu", "kernel": "k_chi2_plain (full FP64 scan)", "achieved": achieved,
+                "peak": PEAK_FP64_VALU, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU, "traffic": None,
+                "work_per_launch_flop": flops, "avg_launch_ms": k_ms}
+    else:
+        k_ms = stages.get("scan", float("nan"))
+        flops = scan_flops(cn, n_stars)
+        achieved = flops / (k_ms / 1000.0) / 1e12
+        roof = {"bound": "valu", "kernel": "fast scan (effective: full-scan FLOP count / time)",
+                "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
+                "traffic": None, "work_per_launch_flop": flops, "avg_launch_ms": k_ms}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(clusters, stars, threads)
+
+    value = pairs / (ms / 1000.0)
+    line = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if form == hip.PLAIN else "f32+f64", "data": "synthetic",
+        "config": {"workload": "screen_20k_x_4087_with_shuffled_control", "stars": len(stars), "control_stars":
+                   len(control), "clusters": len(clusters), "pairs_per_step": pairs, "chi2_max": chi2_max,
+                   "n_samp": 64, "r_kernel": 2.0, "p_min": p_min, "form": result["form"],
+                   "cluster_tables_s": tables_s},
+        "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if plain:
+        line["plain"] = plain
+        line["speedup_vs_plain"] = plain["ms"] / ms
+    print(json.dumps(line), flush=True)
+    e.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

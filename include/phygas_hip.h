@@ -1,0 +1,123 @@
+/* C ABI of the MI355X (gfx950) implementation of the Phygas trace-back path.
+ *
+ * Plain C: pointers, sizes and the record types of phygas_types.h.  Two layers:
+ *
+ *  1. pg_*  -- drop-in for the reference's GPU backend interface (reference: src/phygas_gpu.h:16-31,
+ *     implemented there by src/phygas_cuda.cu:25-77 and src/phygas_opencl.c).  Same signatures, buffer
+ *     layouts, slicing state and synchronous 0 = ok contract, so the reference's BOINC driver
+ *     (src/main_phygas_boinc.cpp:59-301) links against this library unchanged.  Plain FP64.
+ *
+ *  2. phx_* -- the batched engine: a context per device with the cluster tables resident in HBM, star
+ *     batches screened in one device pass (draws, tracks, scan, 512-draw finish and row compaction all on
+ *     the GPU), Monte Carlo pairs with device-side quantiles, in a plain FP64 form and the fast form.
+ *     Replaces the host loops of src/phygas_screen.c:48-67 and src/main_phygas_boinc.cpp:59-301.
+ *
+ * Every function returns 0 on success and a nonzero code on failure (message: phx_last_error()). */
+#ifndef PHYGAS_HIP_H
+#define PHYGAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "phygas_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- 1. reference GPU backend ABI */
+typedef void *PgBuf;
+
+/* reference: src/phygas_gpu.h:21 (src/phygas_cuda.cu:25-36) */
+int pg_init(int platform, int device);
+/* reference: src/phygas_gpu.h:22-25 (src/phygas_cuda.cu:38-45); 0 bytes allocates 8 */
+PgBuf pg_alloc(size_t bytes);
+int pg_put(PgBuf b, const void *src, size_t bytes);
+int pg_get(PgBuf b, void *dst, size_t bytes);
+void pg_free(PgBuf b);
+/* reference: src/phygas_gpu.h:28-29, device src/phygas_kernel_body.h:64-109 */
+int pg_lin(PgBuf ps0, PgBuf sig, PgBuf state, PgBuf trk, PgBuf params, double eps_pos, double eps_vel, double dt,
+           int sub, int k0, int nk, int npts, int ns);
+/* reference: src/phygas_gpu.h:30-31, device src/phygas_kernel_body.h:124-163 */
+int pg_chi2(PgBuf trk, PgBuf csoa, PgBuf cn, PgBuf R2s, int nb, int nc, int npts, int p0, int np, PgBuf chi2,
+            PgBuf kb, PgBuf sb);
+/* reference: src/phygas_gpu.h:32-33, device src/phygas_kernel_body.h:181-220 */
+int pg_finish(PgBuf trk, PgBuf csoa, PgBuf cn, PgBuf r50s, PgBuf rts, int nc, int npts, PgBuf hs, PgBuf hc, PgBuf hkb,
+              PgBuf hsb, PgBuf hseed, int h0, int nh, PgBuf out);
+/* reference: src/phygas_gpu.h:34, device src/phygas_kernel_body.h:222-274 */
+int pg_pair(PgBuf ps, PgBuf steps, PgBuf dts, PgBuf st, PgBuf res, PgBuf params, int s0, int nk, int n);
+
+/* ---------------------------------------------------------------- 2. batched engine */
+typedef struct PhxCtx PhxCtx;
+
+/* PLAIN: the reference's GPU algorithm in FP64 (full scan, every kept pair finished with FP64 draws).
+ * FAST: FP32-offset tracks, exact culled scan, screened finish (the chosen form).
+ * FAST64: the plain FP64 tracks with FAST's culled scan and screened finish: rows identical to PLAIN's. */
+enum { PHX_FORM_PLAIN = 0, PHX_FORM_FAST = 1, PHX_FORM_FAST64 = 2 };
+
+/* One screened (star, cluster) pair.  flags bit 0: the star belongs to the control group. */
+typedef struct {
+    int star, cluster, kb, flags;
+    double chi2, sb, t, d_nom_pc, sig_pc, p_r50, p_rtot, vrel;
+} PhxRow;
+
+/* Context on a device with its own stream (default priority).  Not thread-safe: one host thread per
+ * context.  No counterpart in the reference (its backend keeps one global device). */
+int phx_create(int device, PhxCtx **out);
+void phx_destroy(PhxCtx *ctx);
+const char *phx_last_error(void);
+int phx_sync(PhxCtx *ctx);
+/* The context's stream (a hipStream_t) for callers that time or order work around it. */
+void *phx_stream(PhxCtx *ctx);
+
+/* Cluster tables: n_samp draws (seed "lin"|name) and the linear tracks of every cluster, computed on the
+ * device and kept resident; r_kernel as the reference's ph_screen_r (0 = total radius).
+ * Replaces the precompute loop of src/phygas_screen.c:32-33 / src/main_phygas_boinc.cpp:402-407 and the
+ * table packing of src/main_phygas_boinc.cpp:61-90. */
+int phx_load_clusters(PhxCtx *ctx, const PhCluster *cl, int nc, int n_samp, double r_kernel);
+/* Star batch resident on the device: append ns records to group (0 = real, 1 = control). */
+int phx_load_stars(PhxCtx *ctx, const PhTarget *st, int ns, int group);
+int phx_clear_stars(PhxCtx *ctx);
+
+/* One screening pass over every loaded star against the resident clusters (reference semantics of
+ * src/phygas_screen.c:50-62 with the work-unit cut p_r50 >= p_min of src/main_phygas_boinc.cpp:441):
+ * asynchronous on the context's stream.  form: PHX_FORM_PLAIN (reference GPU algorithm, FP64) or
+ * PHX_FORM_FAST. */
+int phx_screen(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min);
+/* Waits for the pass and copies its rows out, ordered by (group, star, cluster). *n = rows available
+ * (may exceed cap, then only cap are copied). */
+int phx_rows(PhxCtx *ctx, PhxRow *rows, long cap, long *n);
+/* Pairs that passed the chi2 cut in the last pass (finished pairs). */
+int phx_kept(PhxCtx *ctx, long *n);
+/* Durations (ms, HIP events on the context's stream) of the last pass's stages; returns the count, names
+ * into names[] (static strings). */
+int phx_stage_times(PhxCtx *ctx, double *ms, const char **names, int cap);
+void phx_set_timing(PhxCtx *ctx, int on);
+
+/* Parity access to intermediate device results of the last plain pass / the cluster tables. */
+int phx_cluster_tracks(PhxCtx *ctx, int *n, double *pos, double *vel, double *S, int npts_cap);
+int phx_star_tracks(PhxCtx *ctx, int first, int count, double *trk, int npts_cap);
+int phx_chi2_matrix(PhxCtx *ctx, double *chi2, int *kb, double *sb);
+int phx_npts(PhxCtx *ctx);
+/* Largest error of the FP32 Box-Muller estimate the fast finish screens draws with, over n uniform pairs of
+ * splitmix64(seed) (the screen keeps draws within 1e-3 of a counting interval). */
+int phx_check_finish_screen(PhxCtx *ctx, uint64_t seed, long n, double *worst);
+
+/* Monte Carlo clones already drawn (ps[n][12] = target pos/vel, cluster pos/vel): closest approach (kpc),
+ * its time (Myr ago) and relative speed (kpc/Myr), per clone (reference: src/phygas.c:200-216). */
+int phx_pair_clones(PhxCtx *ctx, int form, const double *ps, const int *steps, const double *dts, int n, double *best,
+                    double *tbest, double *vbest);
+/* Full Monte Carlo pairs (reference: src/phygas.c:240-249): clones drawn on the host with the reference
+ * RNG (seed ph_seed(id, name)), traced on the device, quantiles and fractions reduced on the device. */
+int phx_pair(PhxCtx *ctx, int form, const PhTarget *t, const PhCluster *c, int npairs, const int *ti, const int *ci,
+             int n_clones, double dt_myr, PhResult *res);
+
+/* Host helpers with the reference's semantics. */
+uint64_t phx_seed(const char *a, const char *b); /* reference: src/phygas.c:452-458 */
+int phx_pair_draw(const PhTarget *t, const PhCluster *c, int n_clones, double dt_myr, uint64_t seed, double *ps,
+                  int *steps, double *dt); /* reference: src/phygas.c:182-198 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

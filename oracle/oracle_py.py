@@ -10,7 +10,18 @@ import os
 
 import numpy as np
 
+from phygas_amd.workload import CLUSTER_DTYPE, TARGET_DTYPE
+
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _cl(a) -> np.ndarray:
+    """Cluster records in the C layout of PhCluster (numpy may pack a structured dtype on concatenate)."""
+    return np.ascontiguousarray(np.atleast_1d(a), dtype=CLUSTER_DTYPE)
+
+
+def _tg(a) -> np.ndarray:
+    return np.ascontiguousarray(np.atleast_1d(a), dtype=TARGET_DTYPE)
 
 ROW_DTYPE = np.dtype([("star", "<i4"), ("cluster", "<i4"), ("kb", "<i4"), ("flags", "<i4"), ("chi2", "<f8"),
                       ("sb", "<f8"), ("t", "<f8"), ("d_nom_pc", "<f8"), ("sig_pc", "<f8"), ("p_r50", "<f8"),
@@ -129,27 +140,26 @@ class Oracle:
         off = np.zeros(cnt, np.int64)
         cap = 202 * max(cnt, 1)
         pos, vel, S = np.zeros((cap, 3)), np.zeros((cap, 3)), np.zeros((cap, 6))
-        recs = np.ascontiguousarray(recs)
         tot = fn(_ptr(recs), cnt, *args, nthreads, _ptr(n), _ptr(off), _ptr(pos), _ptr(vel), _ptr(S), cap)
         if tot < 0:
             raise RuntimeError("track buffer too small")
         return n, off, pos[:tot], vel[:tot], S[:tot]
 
     def lin_clusters(self, cl, n_samp=64, nthreads=8):
-        return self._lin(self.lib.orc_lin_clusters, cl, n_samp, nthreads=nthreads)
+        return self._lin(self.lib.orc_lin_clusters, _cl(cl), n_samp, nthreads=nthreads)
 
     def lin_targets(self, tg, tmax, n_samp=64, nthreads=8):
-        return self._lin(self.lib.orc_lin_targets, tg, C.c_double(tmax), n_samp, nthreads=nthreads)
+        return self._lin(self.lib.orc_lin_targets, _tg(tg), C.c_double(tmax), n_samp, nthreads=nthreads)
 
     def lin_init_targets(self, tg, n_samp=64):
-        tg = np.ascontiguousarray(tg)
+        tg = _tg(tg)
         ps0, sig = np.zeros((len(tg), 6)), np.zeros((len(tg), 36))
         self.lib.orc_lin_init_targets(_ptr(tg), len(tg), n_samp, _ptr(ps0), _ptr(sig))
         return ps0, sig
 
     # ---- screening
     def screen(self, cl, tg, chi2_max=25.0, n_samp=64, r_kernel=2.0, use_cut=True, nthreads=8, t0=0, t1=None):
-        cl, tg = np.ascontiguousarray(cl), np.ascontiguousarray(tg)
+        cl, tg = _cl(cl), _tg(tg)
         t1 = len(tg) if t1 is None else t1
         cap = 1 << 16
         while True:
@@ -161,7 +171,7 @@ class Oracle:
             cap = -n
 
     def chi2_matrix(self, cl, tg, n_samp=64, r_kernel=2.0, nthreads=8):
-        cl, tg = np.ascontiguousarray(cl), np.ascontiguousarray(tg)
+        cl, tg = _cl(cl), _tg(tg)
         chi2 = np.zeros((len(tg), len(cl)))
         kb = np.zeros((len(tg), len(cl)), np.int32)
         sb = np.zeros((len(tg), len(cl)))
@@ -171,8 +181,7 @@ class Oracle:
 
     # ---- Monte Carlo pairs
     def pair_draw(self, t, c, n_clones, dt_myr, seed):
-        t = np.ascontiguousarray(np.atleast_1d(t)[:1])
-        c = np.ascontiguousarray(np.atleast_1d(c)[:1])
+        t, c = _tg(t)[:1], _cl(c)[:1]
         ps = np.zeros((max(n_clones, 1), 12))
         steps = np.zeros(1, np.int32)
         dt = np.zeros(1)
@@ -189,20 +198,20 @@ class Oracle:
         return b, t, v
 
     def pair_reduce(self, c, best, tbest, vbest):
-        c = np.ascontiguousarray(np.atleast_1d(c)[:1])
+        c = _cl(c)[:1]
         best, tbest, vbest = (np.ascontiguousarray(a, dtype=np.float64) for a in (best, tbest, vbest))
         r = np.zeros(1, RESULT_DTYPE)
         self.lib.ph_pair_reduce(_ptr(c), len(best), _ptr(best), _ptr(tbest), _ptr(vbest), _ptr(r))
         return r[0]
 
     def prefilter_mask(self, tg, cl):
-        tg, cl = np.ascontiguousarray(tg), np.ascontiguousarray(cl)
+        tg, cl = _tg(tg), _cl(cl)
         m = np.zeros((len(tg), len(cl)), np.uint8)
         self.lib.orc_prefilter_mask(_ptr(tg), len(tg), _ptr(cl), len(cl), _ptr(m))
         return m.astype(bool)
 
     def pair_run(self, tg, cl, ti, ci, n_clones, dt_myr=0.05, nthreads=8):
-        tg, cl = np.ascontiguousarray(tg), np.ascontiguousarray(cl)
+        tg, cl = _tg(tg), _cl(cl)
         ti = np.ascontiguousarray(ti, dtype=np.int32)
         ci = np.ascontiguousarray(ci, dtype=np.int32)
         res = np.zeros(len(ti), RESULT_DTYPE)

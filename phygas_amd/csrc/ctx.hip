@@ -1,0 +1,543 @@
+// Batched engine (phx_* C ABI): one context per device, cluster tables resident in HBM, star batches
+// screened in a single device pass, Monte Carlo pairs traced and reduced on the device.
+//
+// HBM layout of the resident tables (nc clusters, P = ceil(max tmax / 0.5) + 1 points):
+//   csoa [P][12][nc] f64   pos xyz, vel xyz, S (xx xy xz yy yz zz) per point, clusters innermost so a
+//                          wavefront reading 64 neighbouring clusters is one coalesced 512-byte load
+//   cn [nc] i32, R2 / r50 / rtot [nc] f64 (kpc), records [nc] PhCluster (names seed the finish)
+// Star batch: records [ns] PhTarget, "lin"|id seeds [ns] u64, "id|" hash prefixes [ns] u64, group [ns].
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "fast.h"
+
+using namespace phx;
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return 0;
+        if (p) PHX_CK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        size_t b = bytes ? bytes : 8;
+        PHX_CK(hipMalloc(&p, b));
+        cap = b;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+constexpr int MAX_STAGES = 12;
+
+}  // namespace
+
+struct PhxCtx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    Pot P;
+    // clusters
+    int nc = 0, npts = 0, n_samp_c = 64;
+    double r_kernel = 2.0;
+    std::vector<int> cn_h;
+    std::vector<PhCluster> cl_h;
+    DBuf cl, cseed, cps0, csig, csoa, cn, R2, r50, rt;
+    FastTables fast;   // fast-form tables (fast.hip), built on first use
+    bool fast_ready = false;
+    // stars
+    std::vector<PhTarget> st_h;
+    std::vector<int> group_h;
+    bool stars_dirty = true;
+    DBuf sr, sseed, sprefix, sgroup;
+    // plain pass
+    DBuf ps0, sig, trk, chi2, kb, sb, seltmp, idx, counts, fout, ridx, rows;
+    long kept_cap = 0, row_cap = 0;
+    int last_form = -1;
+    double last_chi2_max = 25.0, last_p_min = 0.0;
+    int last_n_samp = 64;
+    FastWork fw;       // fast-form work buffers
+    // timing
+    bool timing = false;
+    int nstage = 0;
+    hipEvent_t ev[MAX_STAGES + 1] = {};
+    const char *stage_name[MAX_STAGES] = {};
+    // pairs
+    DBuf pps, psteps, pdts, pst, pres, poff, pm, pr50, prt, pout;
+};
+
+static uint64_t fnv_str(uint64_t h, const char *s) {
+    for (const unsigned char *p = (const unsigned char *)s; *p; p++) h = (h ^ *p) * FNV_PRIME;
+    return h;
+}
+
+extern "C" uint64_t phx_seed(const char *a, const char *b) {
+    uint64_t h = fnv_str(FNV_BASIS, a);
+    h = (h ^ (unsigned char)'|') * FNV_PRIME;
+    return fnv_str(h, b);
+}
+
+extern "C" const char *phx_last_error(void) { return last_error(); }
+
+extern "C" int phx_create(int device, PhxCtx **out) {
+    *out = nullptr;
+    int n = 0;
+    PHX_CK(hipGetDeviceCount(&n));
+    PHX_REQUIRE(n > 0, "no HIP device");
+    PHX_REQUIRE(device >= 0 && device < n, "device index out of range");
+    PHX_CK(hipSetDevice(device));
+    PhxCtx *c = new PhxCtx();
+    c->device = device;
+    c->P = default_pot();
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("phx_create", "stream creation failed");
+        return 1;
+    }
+    for (int i = 0; i <= MAX_STAGES; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            set_error("phx_create", "event creation failed");
+            return 1;
+        }
+    *out = c;
+    return 0;
+}
+
+extern "C" void phx_destroy(PhxCtx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    DBuf *bufs[] = {&c->cl, &c->cseed, &c->cps0, &c->csig, &c->csoa, &c->cn, &c->R2, &c->r50, &c->rt,
+                    &c->sr, &c->sseed, &c->sprefix, &c->sgroup, &c->ps0, &c->sig, &c->trk, &c->chi2,
+                    &c->kb, &c->sb, &c->seltmp, &c->idx, &c->counts, &c->fout, &c->ridx, &c->rows,
+                    &c->pps, &c->psteps, &c->pdts, &c->pst, &c->pres, &c->poff, &c->pm, &c->pr50,
+                    &c->prt, &c->pout};
+    for (DBuf *b : bufs) b->release();
+    fast_release(c->fast, c->fw);
+    for (int i = 0; i <= MAX_STAGES; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+extern "C" int phx_sync(PhxCtx *c) {
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" void *phx_stream(PhxCtx *c) { return (void *)c->st; }
+
+extern "C" void phx_set_timing(PhxCtx *c, int on) { c->timing = on != 0; }
+
+static void stage_begin(PhxCtx *c) {
+    c->nstage = 0;
+    if (c->timing) (void)hipEventRecord(c->ev[0], c->st);
+}
+static void stage_mark(PhxCtx *c, const char *name) {
+    if (!c->timing || c->nstage >= MAX_STAGES) return;
+    c->stage_name[c->nstage] = name;
+    c->nstage++;
+    (void)hipEventRecord(c->ev[c->nstage], c->st);
+}
+
+extern "C" int phx_stage_times(PhxCtx *c, double *ms, const char **names, int cap) {
+    PHX_CK(hipStreamSynchronize(c->st));
+    int n = std::min(c->nstage, cap);
+    for (int i = 0; i < n; i++) {
+        float f = 0;
+        PHX_CK(hipEventElapsedTime(&f, c->ev[i], c->ev[i + 1]));
+        ms[i] = f;
+        if (names) names[i] = c->stage_name[i];
+    }
+    return n;
+}
+
+// ------------------------------------------------------------------------------------- clusters
+extern "C" int phx_load_clusters(PhxCtx *c, const PhCluster *cl, int nc, int n_samp, double r_kernel) {
+    PHX_REQUIRE(nc > 0, "no clusters");
+    PHX_CK(hipSetDevice(c->device));
+    c->nc = nc;
+    c->n_samp_c = n_samp;
+    c->r_kernel = r_kernel;
+    c->fast_ready = false;
+    c->cl_h.assign(cl, cl + nc);
+    double tmax = 0;
+    for (int i = 0; i < nc; i++) tmax = std::max(tmax, cl[i].tmax_myr);
+    c->npts = (int)ceil(tmax / STEP_MYR) + 1;   // reference: src/main_phygas_boinc.cpp:61
+    c->cn_h.resize(nc);
+    std::vector<uint64_t> seeds(nc);
+    std::vector<double> R2(nc), r50(nc), rt(nc);
+    for (int i = 0; i < nc; i++) {
+        c->cn_h[i] = (int)ceil(cl[i].tmax_myr / STEP_MYR) + 1;   // reference: src/phygas.c:297
+        seeds[i] = phx_seed("lin", cl[i].name);
+        double R = (r_kernel > 0 ? r_kernel * cl[i].r50_pc : cl[i].rtot_pc) / 1000.0;   // src/phygas.c:374
+        R2[i] = R * R;
+        r50[i] = cl[i].r50_pc / 1000.0;
+        rt[i] = cl[i].rtot_pc / 1000.0;
+    }
+    if (c->cl.ensure(sizeof(PhCluster) * nc) || c->cseed.ensure(8 * nc) || c->cps0.ensure(48 * nc) ||
+        c->csig.ensure(288 * nc) || c->csoa.ensure(sizeof(double) * 12 * (size_t)c->npts * nc) ||
+        c->cn.ensure(4 * nc) || c->R2.ensure(8 * nc) || c->r50.ensure(8 * nc) || c->rt.ensure(8 * nc))
+        return 1;
+    PHX_CK(hipMemcpyAsync(c->cl.p, cl, sizeof(PhCluster) * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->cseed.p, seeds.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->cn.p, c->cn_h.data(), 4 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->R2.p, R2.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->r50.p, r50.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->rt.p, rt.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemsetAsync(c->csoa.p, 0, c->csoa.cap, c->st));
+    if (launch_lin_init_clusters(c->cl.as<PhCluster>(), c->cseed.as<uint64_t>(), nc, n_samp, c->P,
+                                 c->cps0.as<double>(), c->csig.as<double>(), c->st))
+        return 1;
+    if (launch_lin_plain(c->cps0.as<double>(), c->csig.as<double>(), nullptr, c->csoa.as<double>(), c->P,
+                         PH_LIN_EPS_POS, PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB, PH_LIN_SUB, 0, c->npts, c->npts, nc,
+                         nc, c->cn.as<int>(), c->st))
+        return 1;
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_npts(PhxCtx *c) { return c->npts; }
+
+extern "C" int phx_cluster_tracks(PhxCtx *c, int *n, double *pos, double *vel, double *S, int npts_cap) {
+    PHX_REQUIRE(c->nc > 0, "no clusters loaded");
+    PHX_REQUIRE(npts_cap >= c->npts, "npts_cap too small");
+    std::vector<double> h(c->csoa.cap / 8);
+    PHX_CK(hipMemcpyAsync(h.data(), c->csoa.p, c->csoa.cap, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    const int nc = c->nc;
+    for (int i = 0; i < nc; i++) {
+        n[i] = c->cn_h[i];
+        for (int k = 0; k < c->cn_h[i]; k++) {
+            size_t o = (size_t)i * npts_cap + k;
+            for (int q = 0; q < 3; q++) {
+                pos[o * 3 + q] = h[((size_t)k * 12 + q) * nc + i];
+                vel[o * 3 + q] = h[((size_t)k * 12 + 3 + q) * nc + i];
+            }
+            for (int q = 0; q < 6; q++) S[o * 6 + q] = h[((size_t)k * 12 + 6 + q) * nc + i];
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------- stars
+extern "C" int phx_clear_stars(PhxCtx *c) {
+    c->st_h.clear();
+    c->group_h.clear();
+    c->stars_dirty = true;
+    return 0;
+}
+
+extern "C" int phx_load_stars(PhxCtx *c, const PhTarget *s, int ns, int group) {
+    PHX_REQUIRE(ns >= 0, "negative star count");
+    c->st_h.insert(c->st_h.end(), s, s + ns);
+    c->group_h.insert(c->group_h.end(), (size_t)ns, group);
+    c->stars_dirty = true;
+    return 0;
+}
+
+static int upload_stars(PhxCtx *c) {
+    if (!c->stars_dirty) return 0;
+    const int ns = (int)c->st_h.size();
+    std::vector<uint64_t> seed(ns), prefix(ns);
+    for (int i = 0; i < ns; i++) {
+        seed[i] = phx_seed("lin", c->st_h[i].id);
+        prefix[i] = seed_prefix(c->st_h[i].id);
+    }
+    if (c->sr.ensure(sizeof(PhTarget) * ns) || c->sseed.ensure(8 * ns) || c->sprefix.ensure(8 * ns) ||
+        c->sgroup.ensure(4 * ns))
+        return 1;
+    PHX_CK(hipMemcpyAsync(c->sr.p, c->st_h.data(), sizeof(PhTarget) * ns, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->sseed.p, seed.data(), 8 * ns, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->sprefix.p, prefix.data(), 8 * ns, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->sgroup.p, c->group_h.data(), 4 * ns, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    c->stars_dirty = false;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------- screening
+static int fast_prepare(PhxCtx *c) {
+    if (c->fast_ready) return 0;
+    if (fast_build_tables(c->fast, c->csoa.as<double>(), c->cn.as<int>(), c->R2.as<double>(), c->nc, c->npts, c->st))
+        return 1;
+    c->fast_ready = true;
+    return 0;
+}
+
+static int screen_plain(PhxCtx *c, double chi2_max, int n_samp, double p_min) {
+    const int ns = (int)c->st_h.size(), nc = c->nc, npts = c->npts;
+    const long npairs = (long)ns * nc;
+    if (c->kept_cap == 0) c->kept_cap = std::max(1L << 20, npairs / 8);
+    if (c->row_cap == 0) c->row_cap = std::max(1L << 16, c->kept_cap / 16);
+    c->kept_cap = std::min(c->kept_cap, std::max(npairs, 1L));
+    if (c->ps0.ensure(48 * (size_t)ns) || c->sig.ensure(288 * (size_t)ns) ||
+        c->trk.ensure(sizeof(double) * 12 * (size_t)npts * ns) || c->chi2.ensure(8 * (size_t)npairs) ||
+        c->kb.ensure(4 * (size_t)npairs) || c->sb.ensure(8 * (size_t)npairs) ||
+        c->seltmp.ensure(select_tmp_bytes(std::max(npairs, c->kept_cap))) || c->idx.ensure(8 * (size_t)c->kept_cap) ||
+        c->counts.ensure(16) || c->fout.ensure(48 * (size_t)c->kept_cap) || c->ridx.ensure(8 * (size_t)c->row_cap) ||
+        c->rows.ensure(sizeof(PhxRow) * (size_t)c->row_cap))
+        return 1;
+    long *cnt = c->counts.as<long>();
+    stage_begin(c);
+    if (launch_lin_init_targets(c->sr.as<PhTarget>(), c->sseed.as<uint64_t>(), ns, n_samp, c->P, c->ps0.as<double>(),
+                                c->sig.as<double>(), c->st))
+        return 1;
+    stage_mark(c, "draws");
+    if (launch_lin_plain(c->ps0.as<double>(), c->sig.as<double>(), nullptr, c->trk.as<double>(), c->P, PH_LIN_EPS_POS,
+                         PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB, PH_LIN_SUB, 0, npts, npts, ns, 0, nullptr, c->st))
+        return 1;
+    stage_mark(c, "tracks");
+    if (launch_chi2_plain(c->trk.as<double>(), c->csoa.as<double>(), c->cn.as<int>(), c->R2.as<double>(), ns, nc, npts,
+                          0, npairs, c->chi2.as<double>(), c->kb.as<int>(), c->sb.as<double>(), c->st))
+        return 1;
+    stage_mark(c, "scan");
+    if (launch_select_less(c->chi2.as<double>(), npairs, chi2_max, c->idx.as<long>(), c->kept_cap, cnt,
+                           c->seltmp.p, c->st))
+        return 1;
+    stage_mark(c, "select");
+    if (launch_finish_sel_plain(c->trk.as<double>(), c->csoa.as<double>(), c->cn.as<int>(), c->r50.as<double>(),
+                                c->rt.as<double>(), c->cl.as<PhCluster>(), c->sprefix.as<uint64_t>(), nc, npts,
+                                c->idx.as<long>(), cnt, c->kept_cap, c->kb.as<int>(), c->sb.as<double>(),
+                                c->fout.as<double>(), c->st))
+        return 1;
+    stage_mark(c, "finish");
+    if (launch_select_geq_counted(c->fout.as<double>() + 3, 6, cnt, c->kept_cap, p_min, c->ridx.as<long>(), c->row_cap,
+                                  cnt + 1, c->seltmp.p, c->st) ||
+        launch_rows(c->idx.as<long>(), c->ridx.as<long>(), cnt + 1, c->row_cap, nc, c->sgroup.as<int>(),
+                    c->chi2.as<double>(), c->kb.as<int>(), c->sb.as<double>(), c->fout.as<double>(),
+                    c->rows.as<PhxRow>(), c->st))
+        return 1;
+    stage_mark(c, "rows");
+    return 0;
+}
+
+static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min) {
+    if (form == PHX_FORM_PLAIN) return screen_plain(c, chi2_max, n_samp, p_min);
+    if (fast_prepare(c)) return 1;
+    stage_begin(c);
+    FastArgs a;
+    a.P = c->P; a.ns = (int)c->st_h.size(); a.nc = c->nc; a.npts = c->npts; a.n_samp = n_samp;
+    a.chi2_max = chi2_max; a.p_min = p_min; a.exact_tracks = form == PHX_FORM_FAST64;
+    a.stars = c->sr.as<PhTarget>(); a.seeds = c->sseed.as<uint64_t>(); a.prefix = c->sprefix.as<uint64_t>();
+    a.group = c->sgroup.as<int>(); a.cl = c->cl.as<PhCluster>(); a.csoa = c->csoa.as<double>();
+    a.cn = c->cn.as<int>(); a.R2 = c->R2.as<double>(); a.r50 = c->r50.as<double>(); a.rt = c->rt.as<double>();
+    auto mark = [](void *ctx, const char *name) { stage_mark((PhxCtx *)ctx, name); };
+    return fast_screen(c->fast, c->fw, a, c->st, mark, c);
+}
+
+extern "C" int phx_screen(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min) {
+    PHX_REQUIRE(c->nc > 0, "load clusters first");
+    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "unknown form");
+    PHX_CK(hipSetDevice(c->device));
+    if (upload_stars(c)) return 1;
+    c->last_form = form;
+    c->last_chi2_max = chi2_max;
+    c->last_n_samp = n_samp;
+    c->last_p_min = p_min;
+    if (c->st_h.empty()) return 0;
+    return screen_pass(c, form, chi2_max, n_samp, p_min);
+}
+
+// Kept / row counts of the last pass; reruns the pass with larger buffers when a selection overflowed.
+static int last_counts(PhxCtx *c, long *kept, long *nrows) {
+    if (c->st_h.empty() || c->last_form < 0) {
+        *kept = *nrows = 0;
+        return 0;
+    }
+    for (int attempt = 0; attempt < 4; attempt++) {
+        long h[2];
+        if (c->last_form == PHX_FORM_PLAIN) {
+            PHX_CK(hipMemcpyAsync(h, c->counts.p, sizeof h, hipMemcpyDeviceToHost, c->st));
+            PHX_CK(hipStreamSynchronize(c->st));
+            if (h[0] > c->kept_cap || h[1] > c->row_cap) {
+                c->kept_cap = std::max(c->kept_cap, h[0] + h[0] / 4 + 1024);
+                c->row_cap = std::max(c->row_cap, h[1] + h[1] / 4 + 1024);
+                if (screen_plain(c, c->last_chi2_max, c->last_n_samp, c->last_p_min)) return 1;
+                continue;
+            }
+        } else {
+            int grow = 0;
+            if (fast_counts(c->fw, c->st, h, &grow)) return 1;
+            if (grow) {
+                if (screen_pass(c, c->last_form, c->last_chi2_max, c->last_n_samp, c->last_p_min)) return 1;
+                continue;
+            }
+        }
+        *kept = h[0];
+        *nrows = h[1];
+        return 0;
+    }
+    set_error("phx_rows", "selection buffers kept overflowing");
+    return 3;
+}
+
+extern "C" int phx_kept(PhxCtx *c, long *n) {
+    long r;
+    return last_counts(c, n, &r);
+}
+
+extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
+    long kept, nr;
+    if (last_counts(c, &kept, &nr)) return 1;
+    *n = nr;
+    long m = std::min(cap, nr);
+    if (m > 0) {
+        const void *src = c->last_form == PHX_FORM_PLAIN ? c->rows.p : fast_rows_ptr(c->fw);
+        PHX_CK(hipMemcpyAsync(rows, src, sizeof(PhxRow) * m, hipMemcpyDeviceToHost, c->st));
+        PHX_CK(hipStreamSynchronize(c->st));
+        if (c->last_form != PHX_FORM_PLAIN) std::sort(rows, rows + m, [](const PhxRow &a, const PhxRow &b) {
+            return a.star != b.star ? a.star < b.star : a.cluster < b.cluster;
+        });
+    }
+    return 0;
+}
+
+extern "C" int phx_star_tracks(PhxCtx *c, int first, int count, double *trk, int npts_cap) {
+    PHX_REQUIRE(c->last_form >= 0, "no pass has run");
+    PHX_REQUIRE(npts_cap == c->npts, "npts_cap must equal phx_npts()");
+    PHX_REQUIRE(first >= 0 && first + count <= (int)c->st_h.size(), "star range out of the batch");
+    const double *src = c->last_form == PHX_FORM_PLAIN ? c->trk.as<double>() : fast_tracks_ptr(c->fw);
+    PHX_CK(hipMemcpyAsync(trk, src + (size_t)first * c->npts * 12,
+                          sizeof(double) * 12 * (size_t)c->npts * count, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_chi2_matrix(PhxCtx *c, double *chi2, int *kb, double *sb) {
+    PHX_REQUIRE(c->last_form == PHX_FORM_PLAIN, "the chi2 matrix is kept by the plain pass");
+    size_t n = (size_t)c->st_h.size() * c->nc;
+    PHX_CK(hipMemcpyAsync(chi2, c->chi2.p, 8 * n, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipMemcpyAsync(kb, c->kb.p, 4 * n, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipMemcpyAsync(sb, c->sb.p, 8 * n, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_check_finish_screen(PhxCtx *c, uint64_t seed, long n, double *worst) {
+    PHX_CK(hipSetDevice(c->device));
+    return fast_z0_check(seed, n, worst, c->st);
+}
+
+// ------------------------------------------------------------------------------------- Monte Carlo pairs
+// Clone draws on the host with the reference RNG (reference: src/phygas.c:182-198); the physics is the
+// same FP64 code the device runs, compiled for the host against the C library's libm.
+extern "C" int phx_pair_draw(const PhTarget *t, const PhCluster *cl, int n_clones, double dt_myr, uint64_t seed,
+                             double *ps, int *steps_out, double *dt_out) {
+    const Pot P = default_pot();
+    uint64_t st = seed;
+    int steps = (int)ceil(cl->tmax_myr / dt_myr);
+    *steps_out = steps;
+    *dt_out = -cl->tmax_myr / steps;
+    int m = 0;
+    for (int k = 0; k < n_clones; k++) {
+        V3 ap, av, bp, bv;
+        if (sample_target(P, *t, &st, ap, av) || sample_cluster(P, *cl, &st, bp, bv)) continue;
+        double *o = ps + (size_t)m * 12;
+        o[0] = ap.x; o[1] = ap.y; o[2] = ap.z; o[3] = av.x; o[4] = av.y; o[5] = av.z;
+        o[6] = bp.x; o[7] = bp.y; o[8] = bp.z; o[9] = bv.x; o[10] = bv.y; o[11] = bv.z;
+        m++;
+    }
+    return m;
+}
+
+static int trace_clones(PhxCtx *c, int form, int n, int smax) {
+    if (form == PHX_FORM_PLAIN)
+        return launch_pair_plain(c->pps.as<double>(), c->psteps.as<int>(), c->pdts.as<double>(), c->pst.as<double>(),
+                                 c->pres.as<double>(), c->P, 0, smax, n, c->st);
+    return fast_pair_clones(c->pps.as<double>(), c->psteps.as<int>(), c->pdts.as<double>(), c->pres.as<double>(), c->P,
+                            n, c->st);
+}
+
+extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int *steps, const double *dts, int n,
+                               double *best, double *tbest, double *vbest) {
+    PHX_REQUIRE(n >= 0, "negative clone count");
+    PHX_CK(hipSetDevice(c->device));
+    if (n == 0) return 0;
+    if (c->pps.ensure(96 * (size_t)n) || c->psteps.ensure(4 * (size_t)n) || c->pdts.ensure(8 * (size_t)n) ||
+        c->pst.ensure(168 * (size_t)n) || c->pres.ensure(24 * (size_t)n))
+        return 1;
+    int smax = 0;
+    for (int i = 0; i < n; i++) smax = std::max(smax, steps[i]);
+    PHX_CK(hipMemcpyAsync(c->pps.p, ps, 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->psteps.p, steps, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pdts.p, dts, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    stage_begin(c);
+    if (trace_clones(c, form, n, smax)) return 1;
+    stage_mark(c, "clones");
+    std::vector<double> r(3 * (size_t)n);
+    PHX_CK(hipMemcpyAsync(r.data(), c->pres.p, 24 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; i++) {
+        best[i] = r[3 * i];
+        tbest[i] = r[3 * i + 1];
+        vbest[i] = r[3 * i + 2];
+    }
+    return 0;
+}
+
+extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
+                        const int *ci, int n_clones, double dt_myr, PhResult *res) {
+    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
+    PHX_REQUIRE(n_clones <= 4096, "at most 4096 clones per pair (device reduction size)");
+    PHX_CK(hipSetDevice(c->device));
+    if (npairs == 0) return 0;
+    std::vector<double> ps((size_t)npairs * n_clones * 12);
+    std::vector<int> steps((size_t)npairs * n_clones), m(npairs);
+    std::vector<double> dts((size_t)npairs * n_clones), r50(npairs), rt(npairs);
+    std::vector<long> off(npairs);
+    long n = 0;
+    int smax = 0;
+    for (int p = 0; p < npairs; p++) {
+        const PhTarget &T = t[ti[p]];
+        const PhCluster &C = cl[ci[p]];
+        int st;
+        double dt;
+        off[p] = n;
+        m[p] = phx_pair_draw(&T, &C, n_clones, dt_myr, phx_seed(T.id, C.name), ps.data() + (size_t)n * 12, &st, &dt);
+        for (int k = 0; k < m[p]; k++) {
+            steps[n + k] = st;
+            dts[n + k] = dt;
+        }
+        smax = std::max(smax, st);
+        n += m[p];
+        r50[p] = C.r50_pc;
+        rt[p] = C.rtot_pc;
+    }
+    if (c->pps.ensure(96 * (size_t)std::max(n, 1L)) || c->psteps.ensure(4 * (size_t)std::max(n, 1L)) ||
+        c->pdts.ensure(8 * (size_t)std::max(n, 1L)) || c->pst.ensure(168 * (size_t)std::max(n, 1L)) ||
+        c->pres.ensure(24 * (size_t)std::max(n, 1L)) || c->poff.ensure(8 * (size_t)npairs) ||
+        c->pm.ensure(4 * (size_t)npairs) || c->pr50.ensure(8 * (size_t)npairs) || c->prt.ensure(8 * (size_t)npairs) ||
+        c->pout.ensure(sizeof(PhResult) * (size_t)npairs))
+        return 1;
+    PHX_CK(hipMemcpyAsync(c->pps.p, ps.data(), 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->psteps.p, steps.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pdts.p, dts.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->poff.p, off.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pm.p, m.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pr50.p, r50.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->prt.p, rt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    stage_begin(c);
+    if (n > 0 && trace_clones(c, form, (int)n, smax)) return 1;
+    stage_mark(c, "clones");
+    if (launch_pair_reduce(c->pres.as<double>(), c->poff.as<long>(), c->pm.as<int>(), c->pr50.as<double>(),
+                           c->prt.as<double>(), npairs, c->pout.as<PhResult>(), c->st))
+        return 1;
+    stage_mark(c, "reduce");
+    PHX_CK(hipMemcpyAsync(res, c->pout.p, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}

@@ -1,0 +1,57 @@
+// Fast form of the screening pass and the Monte Carlo clones (fast.hip): interface used by ctx.hip.
+#pragma once
+
+#include "common.h"
+
+namespace phx {
+
+// Per-device tables derived from the resident cluster tracks (built once per cluster load).
+struct FastTables {
+    void *mem[16] = {};
+    size_t bytes[16] = {};
+    int nc = 0, npts = 0, ntiles = 0, nc_pad = 0;
+    double chi2_built = -1.0;          // chi2_max the cluster tiles were built for
+    const int *d_cn = nullptr;         // resident cluster point counts and kernel radii (owned by the context)
+    const double *d_R2 = nullptr;
+};
+
+// Work buffers of one pass (grown on demand, reused across passes).
+struct FastWork {
+    void *mem[16] = {};
+    size_t bytes[16] = {};
+    long cand_cap = 0, kept_cap = 0, row_cap = 0;
+    long last_cand = 0;
+    int scan_blocks = 0;               // grid of the persistent tile scan (0: default)
+};
+
+struct FastArgs {
+    Pot P;
+    int ns, nc, npts, n_samp;
+    double chi2_max, p_min;
+    int exact_tracks;                  // 1: the plain FP64 tracks instead of the FP32-offset tracks
+    const PhTarget *stars;
+    const uint64_t *seeds, *prefix;
+    const int *group;
+    const PhCluster *cl;
+    const double *csoa;
+    const int *cn;
+    const double *R2, *r50, *rt;
+};
+
+typedef void (*StageMark)(void *ctx, const char *name);
+
+int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, const double *d_R2, int nc, int npts,
+                      hipStream_t st);
+int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx);
+// Kept-pair and row counts of the last pass into h[0], h[1]; *grow = 1 when a buffer overflowed (it has
+// been enlarged and the pass must be run again).
+int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow);
+const void *fast_rows_ptr(const FastWork &W);
+const double *fast_tracks_ptr(const FastWork &W);
+// Largest error of the FP32 Box-Muller estimate used to screen draws, over n uniforms pairs.
+int fast_z0_check(uint64_t seed, long n, double *worst, hipStream_t st);
+void fast_release(FastTables &T, FastWork &W);
+int fast_pair_clones(const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P, int n,
+                     hipStream_t st);
+
+}  // namespace phx

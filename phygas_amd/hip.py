@@ -1,0 +1,242 @@
+"""ctypes binding of the HIP library (libphygas_hip.so, C ABI of include/phygas_hip.h).
+
+The host-side mirror of the reference's interfaces for this path:
+
+* ``Engine`` -- the batched engine (phx_*): resident cluster tables, star batches screened on the GPU in one
+  pass, Monte Carlo pairs with device quantiles.  Method names follow the reference operations they replace
+  (``screen`` = the phygas_screen loop, src/phygas_screen.c:48-67; ``pair`` = ph_pair, src/phygas.c:240-249).
+* ``pg_*`` -- the reference's GPU backend ABI (src/phygas_gpu.h:16-31), synchronous.
+
+There is no CPU fallback: loading fails with an error when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .workload import CLUSTER_DTYPE, TARGET_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphygas_hip.so")
+
+PLAIN, FAST, FAST64 = 0, 1, 2
+
+ROW_DTYPE = np.dtype([("star", "<i4"), ("cluster", "<i4"), ("kb", "<i4"), ("flags", "<i4"), ("chi2", "<f8"),
+                      ("sb", "<f8"), ("t", "<f8"), ("d_nom_pc", "<f8"), ("sig_pc", "<f8"), ("p_r50", "<f8"),
+                      ("p_rtot", "<f8"), ("vrel", "<f8")])
+RESULT_DTYPE = np.dtype({"names": ["n", "d05", "d50", "d95", "t05", "t50", "t95", "v05", "v50", "v95", "f_r50",
+                                   "f_rtot"],
+                         "formats": ["<i4"] + ["<f8"] * 11,
+                         "offsets": [0] + [8 + 8 * i for i in range(11)], "itemsize": 96})
+
+_P, _D, _I, _L = C.c_void_p, C.c_double, C.c_int, C.c_long
+_lib = None
+
+
+class PhygasError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded HIP library; raises when it is missing (build it with ``python -m phygas_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PhygasError(f"{LIB_PATH} is missing: build it with `python -m phygas_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pg_init": (_I, [_I, _I]), "pg_alloc": (_P, [C.c_size_t]), "pg_put": (_I, [_P, _P, C.c_size_t]),
+        "pg_get": (_I, [_P, _P, C.c_size_t]), "pg_free": (None, [_P]),
+        "pg_lin": (_I, [_P, _P, _P, _P, _P, _D, _D, _D, _I, _I, _I, _I, _I]),
+        "pg_chi2": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P]),
+        "pg_finish": (_I, [_P, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _I, _P]),
+        "pg_pair": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I]),
+        "phx_create": (_I, [_I, C.POINTER(_P)]), "phx_destroy": (None, [_P]),
+        "phx_last_error": (C.c_char_p, []), "phx_sync": (_I, [_P]), "phx_stream": (_P, [_P]),
+        "phx_load_clusters": (_I, [_P, _P, _I, _I, _D]), "phx_load_stars": (_I, [_P, _P, _I, _I]),
+        "phx_clear_stars": (_I, [_P]), "phx_screen": (_I, [_P, _I, _D, _I, _D]),
+        "phx_rows": (_I, [_P, _P, _L, C.POINTER(_L)]), "phx_kept": (_I, [_P, C.POINTER(_L)]),
+        "phx_stage_times": (_I, [_P, _P, _P, _I]), "phx_set_timing": (None, [_P, _I]),
+        "phx_cluster_tracks": (_I, [_P, _P, _P, _P, _P, _I]), "phx_star_tracks": (_I, [_P, _I, _I, _P, _I]),
+        "phx_chi2_matrix": (_I, [_P, _P, _P, _P]), "phx_npts": (_I, [_P]),
+        "phx_pair_clones": (_I, [_P, _I, _P, _P, _P, _I, _P, _P, _P]),
+        "phx_pair": (_I, [_P, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
+        "phx_seed": (C.c_uint64, [C.c_char_p, C.c_char_p]),
+        "phx_pair_draw": (_I, [_P, _P, _I, _D, C.c_uint64, _P, _P, _P]),
+        "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().phx_last_error()
+        raise PhygasError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def seed(a: str | bytes, b: str | bytes) -> int:
+    """ph_seed (reference: src/phygas.c:452-458)."""
+    a = a.encode() if isinstance(a, str) else a
+    b = b.encode() if isinstance(b, str) else b
+    return int(lib().phx_seed(a, b))
+
+
+def pair_draw(t, c, n_clones: int, dt_myr: float, seed_: int):
+    """ph_pair_draw on the host (reference: src/phygas.c:182-198): (ps[m][12], steps, dt)."""
+    t = np.ascontiguousarray(np.atleast_1d(t)[:1], dtype=TARGET_DTYPE)
+    c = np.ascontiguousarray(np.atleast_1d(c)[:1], dtype=CLUSTER_DTYPE)
+    ps = np.zeros((max(n_clones, 1), 12))
+    steps = np.zeros(1, np.int32)
+    dt = np.zeros(1)
+    m = lib().phx_pair_draw(_p(t), _p(c), n_clones, dt_myr, seed_, _p(ps), _p(steps), _p(dt))
+    return ps[:m], int(steps[0]), float(dt[0])
+
+
+class Engine:
+    """A device context with resident cluster tables (phx_* ABI)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = _P()
+        _check(self._L.phx_create(device, C.byref(h)), "phx_create")
+        self.h = h
+        self.nc = 0
+        self.ns = 0
+        self.clusters = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.phx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- tables and batches
+    def load_clusters(self, cl: np.ndarray, n_samp: int = 64, r_kernel: float = 2.0):
+        cl = np.ascontiguousarray(cl, dtype=CLUSTER_DTYPE)
+        _check(self._L.phx_load_clusters(self.h, _p(cl), len(cl), n_samp, r_kernel), "phx_load_clusters")
+        self.nc = len(cl)
+        self.clusters = cl
+
+    def load_stars(self, st: np.ndarray, group: int = 0):
+        st = np.ascontiguousarray(st, dtype=TARGET_DTYPE)
+        _check(self._L.phx_load_stars(self.h, _p(st), len(st), group), "phx_load_stars")
+        self.ns += len(st)
+
+    def clear_stars(self):
+        _check(self._L.phx_clear_stars(self.h), "phx_clear_stars")
+        self.ns = 0
+
+    @property
+    def npts(self) -> int:
+        return self._L.phx_npts(self.h)
+
+    # ---- screening
+    def screen_async(self, form: int = PLAIN, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0):
+        _check(self._L.phx_screen(self.h, form, chi2_max, n_samp, p_min), "phx_screen")
+
+    def rows(self) -> np.ndarray:
+        n = _L(0)
+        _check(self._L.phx_rows(self.h, None, 0, C.byref(n)), "phx_rows")
+        out = np.zeros(n.value, ROW_DTYPE)
+        if n.value:
+            _check(self._L.phx_rows(self.h, _p(out), n.value, C.byref(n)), "phx_rows")
+        return out
+
+    def screen(self, form: int = PLAIN, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> np.ndarray:
+        self.screen_async(form, chi2_max, n_samp, p_min)
+        return self.rows()
+
+    def kept(self) -> int:
+        n = _L(0)
+        _check(self._L.phx_kept(self.h, C.byref(n)), "phx_kept")
+        return n.value
+
+    def sync(self):
+        _check(self._L.phx_sync(self.h), "phx_sync")
+
+    def set_timing(self, on: bool = True):
+        self._L.phx_set_timing(self.h, int(on))
+
+    def stage_times(self) -> dict[str, float]:
+        ms = np.zeros(16)
+        names = (C.c_char_p * 16)()
+        n = self._L.phx_stage_times(self.h, _p(ms), C.cast(names, _P), 16)
+        if n < 0:
+            raise PhygasError("phx_stage_times failed")
+        return {names[i].decode(): float(ms[i]) for i in range(n)}
+
+    def stream(self) -> int:
+        return self._L.phx_stream(self.h)
+
+    # ---- parity access
+    def cluster_tracks(self):
+        npts = self.npts
+        n = np.zeros(self.nc, np.int32)
+        pos = np.zeros((self.nc, npts, 3))
+        vel = np.zeros((self.nc, npts, 3))
+        S = np.zeros((self.nc, npts, 6))
+        _check(self._L.phx_cluster_tracks(self.h, _p(n), _p(pos), _p(vel), _p(S), npts), "phx_cluster_tracks")
+        return n, pos, vel, S
+
+    def star_tracks(self, first: int = 0, count: int | None = None):
+        count = self.ns - first if count is None else count
+        trk = np.zeros((count, self.npts, 12))
+        _check(self._L.phx_star_tracks(self.h, first, count, _p(trk), self.npts), "phx_star_tracks")
+        return trk
+
+    def chi2_matrix(self):
+        chi2 = np.zeros((self.ns, self.nc))
+        kb = np.zeros((self.ns, self.nc), np.int32)
+        sb = np.zeros((self.ns, self.nc))
+        _check(self._L.phx_chi2_matrix(self.h, _p(chi2), _p(kb), _p(sb)), "phx_chi2_matrix")
+        return chi2, kb, sb
+
+    def check_finish_screen(self, seed: int = 12345, n: int = 1 << 22) -> float:
+        w = np.zeros(1)
+        _check(self._L.phx_check_finish_screen(self.h, seed, n, _p(w)), "phx_check_finish_screen")
+        return float(w[0])
+
+    # ---- Monte Carlo pairs
+    def pair_clones(self, ps, steps, dts, form: int = PLAIN):
+        ps = np.ascontiguousarray(ps, dtype=np.float64)
+        n = len(ps)
+        steps = np.ascontiguousarray(np.broadcast_to(steps, (n,)), dtype=np.int32)
+        dts = np.ascontiguousarray(np.broadcast_to(dts, (n,)), dtype=np.float64)
+        b, t, v = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(self._L.phx_pair_clones(self.h, form, _p(ps), _p(steps), _p(dts), n, _p(b), _p(t), _p(v)),
+               "phx_pair_clones")
+        return b, t, v
+
+    def pair(self, targets, clusters, ti, ci, n_clones: int, dt_myr: float = 0.05, form: int = PLAIN) -> np.ndarray:
+        targets = np.ascontiguousarray(targets, dtype=TARGET_DTYPE)
+        clusters = np.ascontiguousarray(clusters, dtype=CLUSTER_DTYPE)
+        ti = np.ascontiguousarray(ti, dtype=np.int32)
+        ci = np.ascontiguousarray(ci, dtype=np.int32)
+        res = np.zeros(len(ti), RESULT_DTYPE)
+        _check(self._L.phx_pair(self.h, form, _p(targets), _p(clusters), len(ti), _p(ti), _p(ci), n_clones, dt_myr,
+                                _p(res)), "phx_pair")
+        return res

@@ -1,0 +1,151 @@
+"""GPU parity of the fast forms.
+
+* FAST64 (plain FP64 tracks + culled scan + screened finish) must give rows identical to PLAIN, byte for
+  byte: this pins the exactness of the tile/point culling and of the draw screening of the finish.
+* FAST (FP32-offset tracks) against the oracle: positions within 1e-9 relative, covariances within 2e-5
+  (relative to sqrt(S_ii S_jj)), chi2 within 1e-4 relative; p_r50 / p_rtot counts identical for every kept
+  pair on these inputs (the north-star bar is |dp_r50| <= 1e-4, i.e. the same count); kept sets equal except
+  pairs within 1e-4 of the chi2 cut.  Measured errors go to gpurun_out/parity_fast.json.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from phygas_amd import hip
+from phygas_amd import io as pio
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INP = os.path.join(ROOT, "tests", "golden", "inputs")
+REPORT = {}
+
+
+def _report(key, value):
+    REPORT[key] = float(value)
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", "parity_fast.json"), "w") as f:
+        json.dump(REPORT, f, indent=1)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return pio.read_clusters(os.path.join(INP, "clusters400.txt")), pio.read_targets(os.path.join(INP, "stars20.txt"))
+
+
+@pytest.fixture(scope="module")
+def unit(workload):
+    """A bigger sample: 4,087 clusters x 300 stars with their shuffled control."""
+    cl = workload.make_clusters(4087, 0)
+    st = workload.make_stars(300, 2)
+    return cl, st, workload.shuffled_control(st, 3)
+
+
+def test_finish_screen_margin():
+    with hip.Engine(0) as e:
+        worst = e.check_finish_screen(seed=20261203, n=1 << 23)
+    _report("z0_estimate_max_abs_err", worst)
+    assert worst < 1e-4      # the screen keeps draws within 1e-3 of the counting interval
+
+
+@pytest.mark.parametrize("p_min", [0.0, 0.001])
+def test_fast64_rows_identical_to_plain(unit, p_min):
+    cl, st, ctl = unit
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        plain = e.screen(hip.PLAIN, chi2_max=25.0, p_min=p_min)
+        kp = e.kept()
+        fast = e.screen(hip.FAST64, chi2_max=25.0, p_min=p_min)
+        kf = e.kept()
+    _report(f"fast64_rows_p{p_min}", len(fast))
+    assert kp == kf
+    assert len(plain) == len(fast)
+    assert plain.tobytes() == fast.tobytes()
+
+
+def test_fast64_small_chi2_cut_identical(small):
+    cl, st = small
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        for cut in (1.0, 9.0, 100.0):
+            a = e.screen(hip.PLAIN, chi2_max=cut, p_min=0.0)
+            b = e.screen(hip.FAST64, chi2_max=cut, p_min=0.0)
+            assert a.tobytes() == b.tobytes(), cut
+
+
+def test_fast_tracks_match_oracle(small, oracle):
+    cl, st = small
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        e.screen(hip.FAST, chi2_max=25.0)
+        trk = e.star_tracks()
+    n, off, pos, vel, S = oracle.lin_targets(st, float(cl["tmax_myr"].max()))
+    ep = es = 0.0
+    for j in range(len(st)):
+        sl = slice(off[j], off[j] + n[j])
+        ep = max(ep, np.max(np.abs(trk[j, :, :3] - pos[sl]) / np.linalg.norm(pos[sl], axis=1, keepdims=True)))
+        R = S[sl]
+        d = np.sqrt(np.abs(R[:, [0, 0, 0, 3, 3, 5]] * R[:, [0, 3, 5, 3, 5, 5]])) + 1e-300
+        es = max(es, np.max(np.abs(trk[j, :, 6:] - R) / d))
+    _report("fast_track_pos_rel", ep)
+    _report("fast_track_cov_rel", es)
+    assert ep < 1e-9
+    assert es < 2e-5
+
+
+def test_fast_rows_match_oracle(small, oracle):
+    cl, st = small
+    want = oracle.screen(cl, st, chi2_max=25.0)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        got = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+    a = {(r["star"], r["cluster"]): r for r in want}
+    b = {(r["star"], r["cluster"]): r for r in got}
+    for k in set(a) ^ set(b):     # only pairs at the cut may differ
+        r = a.get(k, b.get(k))
+        assert abs(r["chi2"] - 25.0) < 25.0 * 1e-4, (k, r["chi2"])
+    common = sorted(set(a) & set(b))
+    c2 = np.array([abs(a[k]["chi2"] - b[k]["chi2"]) / a[k]["chi2"] for k in common])
+    dp50 = np.array([abs(a[k]["p_r50"] - b[k]["p_r50"]) for k in common])
+    dpt = np.array([abs(a[k]["p_rtot"] - b[k]["p_rtot"]) for k in common])
+    _report("fast_small_chi2_rel_max", c2.max())
+    _report("fast_small_pairs", len(common))
+    _report("fast_small_dp_r50_max", dp50.max())
+    assert c2.max() < 1e-4
+    assert dp50.max() <= 1e-4 and dpt.max() <= 1e-4
+
+
+def test_fast_vs_plain_on_work_unit_sample(unit):
+    """The chosen form against the plain FP64 GPU form on 300 stars + control x 4,087 clusters."""
+    cl, st, ctl = unit
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        plain = e.screen(hip.PLAIN, chi2_max=25.0, p_min=0.0)
+        fast = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+    a = {(r["star"], r["cluster"]): r for r in plain}
+    b = {(r["star"], r["cluster"]): r for r in fast}
+    diff = set(a) ^ set(b)
+    for k in diff:
+        r = a.get(k, b.get(k))
+        assert abs(r["chi2"] - 25.0) < 25.0 * 1e-4
+    common = sorted(set(a) & set(b))
+    dp = np.array([abs(a[k]["p_r50"] - b[k]["p_r50"]) for k in common])
+    dpt = np.array([abs(a[k]["p_rtot"] - b[k]["p_rtot"]) for k in common])
+    c2 = np.array([abs(a[k]["chi2"] - b[k]["chi2"]) / a[k]["chi2"] for k in common])
+    _report("fast_vs_plain_pairs", len(common))
+    _report("fast_vs_plain_only_one_side", len(diff))
+    _report("fast_vs_plain_dp_r50_max", dp.max())
+    _report("fast_vs_plain_frac_dp_r50_above_1e-4", np.mean(dp > 1e-4))
+    _report("fast_vs_plain_frac_dp_rtot_above_1e-4", np.mean(dpt > 1e-4))
+    _report("fast_vs_plain_chi2_rel_max", c2.max())
+    assert np.mean(dp > 1e-4) <= 1e-3
+    assert c2.max() < 1e-4

@@ -1,0 +1,260 @@
+"""GPU parity of the plain FP64 path against the CPU oracle (through the C ABI).
+
+Tolerances (north-star bar for the plain FP64 form): orbits within 1e-9 relative, p_r50 / p_rtot identical
+counts (|dp| <= 1e-4 means the same count of 512), Monte Carlo quantiles within 1e-9 relative here (the bar
+is 1e-3).  Device pow/log/cos/sin differ from glibc in the last bit, so these are tolerances, not bit
+identity.  The covariances come from a finite-difference state-transition matrix, A = (x_j - x_0) / 1e-5,
+which multiplies a one-ulp difference of an 8-kpc position by 1e5: covariance entries (compared relative
+to sqrt(S_ii S_jj)) and the Mahalanobis minima built on them carry that noise floor, COV_REL below; the
+measured maxima are written to gpurun_out/parity_plain.json."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from phygas_amd import hip
+from phygas_amd import io as pio
+from phygas_amd import rows as prows
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INP = os.path.join(ROOT, "tests", "golden", "inputs")
+REL = 1e-9
+COV_REL = 1e-7
+REPORT = {}
+
+
+def _report(key, value):
+    REPORT[key] = max(float(value), REPORT.get(key, 0.0))
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", "parity_plain.json"), "w") as f:
+        json.dump(REPORT, f, indent=1)
+
+
+@pytest.fixture(scope="module")
+def small():
+    cl = pio.read_clusters(os.path.join(INP, "clusters400.txt"))
+    st = pio.read_targets(os.path.join(INP, "stars20.txt"))
+    return cl, st
+
+
+@pytest.fixture(scope="module")
+def engine(small):
+    cl, st = small
+    e = hip.Engine(0)
+    e.load_clusters(cl)
+    e.load_stars(st)
+    e.screen(hip.PLAIN, chi2_max=25.0, p_min=0.0)
+    yield e
+    e.close()
+
+
+def _cov_err(S, R):
+    """max |S - R| / sqrt(R_ii R_jj) over the 6 unique entries (xx xy xz yy yz zz)."""
+    d = np.sqrt(np.abs(R[..., [0, 0, 0, 3, 3, 5]] * R[..., [0, 3, 5, 3, 5, 5]])) + 1e-300
+    return np.max(np.abs(S - R) / d)
+
+
+def test_cluster_tables_match_oracle(engine, small, oracle):
+    cl, _ = small
+    n, off, pos, vel, S = oracle.lin_clusters(cl)
+    gn, gpos, gvel, gS = engine.cluster_tracks()
+    assert np.array_equal(gn, n)
+    for i in range(len(cl)):
+        sl = slice(off[i], off[i] + n[i])
+        scale = np.linalg.norm(pos[sl], axis=1, keepdims=True)
+        ep = np.max(np.abs(gpos[i, :n[i]] - pos[sl]) / scale)
+        ev = np.max(np.abs(gvel[i, :n[i]] - vel[sl]) / np.linalg.norm(vel[sl], axis=1, keepdims=True))
+        es = _cov_err(gS[i, :n[i]], S[sl])
+        _report("cluster_pos_rel", ep)
+        _report("cluster_vel_rel", ev)
+        _report("cluster_cov_rel", es)
+        assert ep < REL and ev < REL
+        assert es < COV_REL, i
+
+
+def test_star_tracks_match_oracle(engine, small, oracle):
+    cl, st = small
+    tmax = float(cl["tmax_myr"].max())
+    n, off, pos, vel, S = oracle.lin_targets(st, tmax)
+    trk = engine.star_tracks()
+    assert trk.shape[1] == n[0]
+    for j in range(len(st)):
+        sl = slice(off[j], off[j] + n[j])
+        ep = np.max(np.abs(trk[j, :, 0:3] - pos[sl]) / np.linalg.norm(pos[sl], axis=1, keepdims=True))
+        es = _cov_err(trk[j, :, 6:12], S[sl])
+        _report("star_pos_rel", ep)
+        _report("star_cov_rel", es)
+        assert ep < REL
+        assert es < COV_REL
+
+
+def test_full_scan_chi2_matches_oracle(engine, small, oracle):
+    cl, st = small
+    chi2, kb, sb = oracle.chi2_matrix(cl, st)
+    g2, gkb, gsb = engine.chi2_matrix()
+    fin = np.isfinite(chi2)
+    assert np.array_equal(fin, np.isfinite(g2))
+    rel = np.abs(g2[fin] - chi2[fin]) / np.maximum(chi2[fin], 1e-300)
+    _report("chi2_rel_max", rel.max())
+    _report("chi2_rel_p99", np.quantile(rel, 0.99))
+    _report("chi2_rel_frac_above_1e-9", np.mean(rel > 1e-9))
+    assert rel.max() < COV_REL, rel.max()
+    # the minimum's position: the same point, except where two points' forms agree to rounding
+    same = (gkb == kb) & (np.abs(gsb - sb) <= 1e-9)
+    _report("kb_sb_mismatch_frac", 1.0 - same.mean())
+    assert same.mean() > 0.999
+
+
+def test_screen_rows_match_oracle(engine, small, oracle):
+    cl, st = small
+    want = oracle.screen(cl, st, chi2_max=25.0, use_cut=True)
+    got = engine.rows()
+    assert len(got) == len(want)
+    assert np.array_equal(got["star"], want["star"]) and np.array_equal(got["cluster"], want["cluster"])
+    assert np.array_equal(got["p_r50"], want["p_r50"]) and np.array_equal(got["p_rtot"], want["p_rtot"])
+    for f in ("t", "d_nom_pc", "vrel"):
+        _report(f"row_{f}_rel", np.max(np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-300)))
+        assert np.allclose(got[f], want[f], rtol=REL, atol=1e-12), f
+    for f in ("chi2", "sig_pc"):
+        _report(f"row_{f}_rel", np.max(np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-300)))
+        assert np.allclose(got[f], want[f], rtol=COV_REL, atol=1e-12), f
+
+
+def test_screen_rows_text_equal_reference(engine, small, golden_dir):
+    cl, st = small
+    got = prows.screen_lines(engine.rows(), st, cl)
+    with open(os.path.join(golden_dir, "rows_screen.txt")) as f:
+        want = f.read().splitlines()
+    assert got == want
+
+
+def test_p_min_cut_and_control_group(small, oracle, workload):
+    cl, st = small
+    ctl = workload.half_file_control(st)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, group=0)
+        e.load_stars(ctl, group=1)
+        rows = e.screen(hip.PLAIN, chi2_max=25.0, p_min=0.001)
+    want_r = oracle.screen(cl, st)
+    want_c = oracle.screen(cl, ctl)
+    want_r, want_c = want_r[want_r["p_r50"] >= 0.001], want_c[want_c["p_r50"] >= 0.001]
+    real, control = rows[rows["flags"] == 0], rows[rows["flags"] == 1]
+    assert np.array_equal(real["cluster"], want_r["cluster"]) and np.array_equal(real["p_r50"], want_r["p_r50"])
+    assert np.array_equal(control["star"] - len(st), want_c["star"])
+    assert np.array_equal(control["p_rtot"], want_c["p_rtot"])
+
+
+def test_pg_abi_slices_match_oracle(small, oracle):
+    """The reference's GPU backend ABI: sliced pg_lin, pg_chi2 over slices, pg_finish, pg_pair."""
+    import ctypes as C
+    L = hip.lib()
+    cl, st = small
+    st = st[:5]
+    assert L.pg_init(0, 0) == 0
+    tmax = float(cl["tmax_myr"].max())
+    npts = int(np.ceil(tmax / 0.5)) + 1
+    ps0, sig = oracle.lin_init_targets(st)
+    nb = len(st)
+    bufs = []
+
+    def put(a):
+        a = np.ascontiguousarray(a)
+        b = L.pg_alloc(a.nbytes)
+        assert b
+        assert L.pg_put(b, a.ctypes.data, a.nbytes) == 0
+        bufs.append(b)
+        return b
+
+    def get(b, a):
+        assert L.pg_get(b, a.ctypes.data, a.nbytes) == 0
+        return a
+
+    d_ps0, d_sig, d_prm = put(ps0), put(sig), put(oracle.params())
+    d_state = put(np.zeros(nb * 63))
+    d_trk = put(np.zeros(nb * npts * 12))
+    k = 0
+    for nk in (4, 16, 60, 1000):       # the reference driver's growing slices
+        if k >= npts:
+            break
+        assert L.pg_lin(d_ps0, d_sig, d_state, d_trk, d_prm, 1e-5, 1e-5, -0.05, 10, k, nk, npts, nb) == 0
+        k += nk
+    trk = get(d_trk, np.zeros((nb, npts, 12)))
+    n, off, pos, vel, S = oracle.lin_targets(st, tmax)
+    for j in range(nb):
+        sl = slice(off[j], off[j] + n[j])
+        assert np.max(np.abs(trk[j, :, :3] - pos[sl]) / np.linalg.norm(pos[sl], axis=1, keepdims=True)) < REL
+    # cluster tables as the reference driver packs them (src/main_phygas_boinc.cpp:63-77)
+    cn_, coff, cpos, cvel, cS = oracle.lin_clusters(cl)
+    nc = len(cl)
+    csoa = np.zeros((npts, 12, nc))
+    for i in range(nc):
+        sl = slice(coff[i], coff[i] + cn_[i])
+        csoa[:cn_[i], 0:3, i], csoa[:cn_[i], 3:6, i], csoa[:cn_[i], 6:12, i] = cpos[sl], cvel[sl], cS[sl]
+    R2 = (2.0 * cl["r50_pc"] / 1000.0) ** 2
+    d_csoa, d_cn, d_R2 = put(csoa), put(cn_.astype(np.int32)), put(R2)
+    d_chi2, d_kb, d_sb = put(np.zeros(nb * nc)), put(np.zeros(nb * nc, np.int32)), put(np.zeros(nb * nc))
+    p = 0
+    for np_ in (1000, 100000):
+        assert L.pg_chi2(d_trk, d_csoa, d_cn, d_R2, nb, nc, npts, p, np_, d_chi2, d_kb, d_sb) == 0
+        p += np_
+    chi2 = get(d_chi2, np.zeros(nb * nc))
+    want, _, _ = oracle.chi2_matrix(cl, st)
+    fin = np.isfinite(want.ravel())
+    assert np.max(np.abs(chi2[fin] - want.ravel()[fin]) / want.ravel()[fin]) < COV_REL
+    # finish the pairs below the cut, in the reference driver's order
+    kb = get(d_kb, np.zeros(nb * nc, np.int32))
+    sb = get(d_sb, np.zeros(nb * nc))
+    sel = np.nonzero(chi2 < 25.0)[0]
+    hs, hc = (sel // nc).astype(np.int32), (sel % nc).astype(np.int32)
+    seeds = np.array([hip.seed(st[s]["id"], cl[c]["name"]) for s, c in zip(hs, hc)], np.uint64)
+    d_out = put(np.zeros(len(sel) * 6))
+    assert L.pg_finish(d_trk, d_csoa, d_cn, put(cl["r50_pc"] / 1000.0), put(cl["rtot_pc"] / 1000.0), nc, npts,
+                       put(hs), put(hc), put(kb[sel]), put(sb[sel]), put(seeds), 0, len(sel), d_out) == 0
+    out = get(d_out, np.zeros((len(sel), 6)))
+    rows = oracle.screen(cl, st)
+    assert np.array_equal(out[:, 3], rows["p_r50"]) and np.array_equal(out[:, 4], rows["p_rtot"])
+    assert np.allclose(out[:, 0], rows["t"], rtol=REL) and np.allclose(out[:, 5], rows["vrel"], rtol=REL)
+    # Monte Carlo clones, sliced in time
+    pul = pio.read_targets(os.path.join(INP, "pulsar1.txt"))
+    ps, steps, dt = oracle.pair_draw(pul, cl[3], 30, 0.05, oracle.seed(pul[0]["id"], cl[3]["name"]))
+    m = len(ps)
+    d_ps, d_steps, d_dts = put(ps), put(np.full(m, steps, np.int32)), put(np.full(m, dt))
+    d_st, d_res = put(np.zeros(m * 21)), put(np.zeros(m * 3))
+    s0 = 0
+    for nk in (20, 80, 320, 5000):
+        if s0 >= steps:
+            break
+        assert L.pg_pair(d_ps, d_steps, d_dts, d_st, d_res, d_prm, s0, nk, m) == 0
+        s0 += nk
+    res = get(d_res, np.zeros((m, 3)))
+    b, t, v = oracle.pair_clones(ps, steps, dt)
+    assert np.allclose(res[:, 0], b, rtol=REL) and np.allclose(res[:, 2], v, rtol=REL)
+    assert np.allclose(res[:, 1], t, rtol=1e-7, atol=1e-9)
+    for b_ in bufs:
+        L.pg_free(b_)
+
+
+def test_pair_plain_matches_oracle_and_reference_rows(small, oracle, golden_dir):
+    cl, _ = small
+    pul = pio.read_targets(os.path.join(INP, "pulsar1.txt"))
+    mask = oracle.prefilter_mask(pul, cl)
+    ti, ci = np.nonzero(mask)
+    want = oracle.pair_run(pul, cl, ti, ci, n_clones=40, dt_myr=0.05)
+    with hip.Engine(0) as e:
+        got = e.pair(pul, cl, ti, ci, n_clones=40, dt_myr=0.05, form=hip.PLAIN)
+    assert np.array_equal(got["n"], want["n"])
+    assert np.array_equal(got["f_r50"], want["f_r50"]) and np.array_equal(got["f_rtot"], want["f_rtot"])
+    for f in ("d05", "d50", "d95", "v05", "v50", "v95", "t05", "t50", "t95"):
+        _report(f"pair_{f}_rel", np.max(np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-300)))
+    for f in ("d05", "d50", "d95", "v05", "v50", "v95"):
+        assert np.allclose(got[f], want[f], rtol=REL), f
+    for f in ("t05", "t50", "t95"):
+        assert np.allclose(got[f], want[f], rtol=1e-7, atol=1e-9), f
+    with open(os.path.join(golden_dir, "rows_pair.txt")) as f_:
+        ref = f_.read().splitlines()
+    lines = prows.pair_lines(got, ti, ci, pul, cl)
+    assert sum(a != b for a, b in zip(lines, ref)) <= 1 and len(lines) == len(ref)

@@ -111,6 +111,48 @@ int phx_pair_clones(PhxCtx *ctx, int form, const double *ps, const int *steps, c
  * RNG (seed ph_seed(id, name)), traced on the device, quantiles and fractions reduced on the device. */
 int phx_pair(PhxCtx *ctx, int form, const PhTarget *t, const PhCluster *c, int npairs, const int *ti, const int *ci,
              int n_clones, double dt_myr, PhResult *res);
+/* phx_pair with options.  PHX_DRAW_ON_DEVICE: the clones are drawn on the GPU from the same splitmix64
+ * stream and in the same order as the reference's host loop (values differ from host draws only by the
+ * device libm's last bits); for batches whose host draws would dominate (thousands of clones per pair). */
+enum { PHX_DRAW_ON_DEVICE = 1 };
+int phx_pair_ex(PhxCtx *ctx, int form, int flags, const PhTarget *t, const PhCluster *c, int npairs, const int *ti,
+                const int *ci, int n_clones, double dt_myr, PhResult *res);
+
+/* ---------------------------------------------------------------- 3. serving loop
+ * Requests (screening chunks, Monte Carlo pair jobs) submitted as they arrive and batched continuously onto
+ * the GPU by slot threads (each with its own stream and resident cluster tables), admitted by HBM footprint.
+ * A request's results are the same whether it runs alone or batched with others.  No counterpart in the
+ * reference (its app runs one work unit per process, src/main_phygas_boinc.cpp:336-471). */
+typedef struct PhxServer PhxServer;
+typedef struct {
+    int form;                 /* PHX_FORM_FAST or PHX_FORM_PLAIN (pairs: FAST = FP32 clones, PLAIN = FP64) */
+    int screen_slots;         /* concurrent screening batches */
+    int pair_slots;           /* concurrent Monte Carlo batches */
+    size_t hbm_budget;        /* bytes the batches in flight may hold by their footprint estimate */
+    long max_batch_stars;     /* stars merged into one screening pass */
+    long max_batch_clones;    /* clones merged into one Monte Carlo pass */
+    double chi2_max, p_min, r_kernel;
+    int n_samp;
+} PhxServeConfig;
+
+void phx_serve_defaults(PhxServeConfig *cfg);
+int phx_server_create(int device, const PhCluster *cl, int nc, const PhxServeConfig *cfg, PhxServer **out);
+/* Stops the slots after the batches in flight; requests still queued are dropped. */
+void phx_server_destroy(PhxServer *srv);
+/* Copies the request; returns its id (>= 0) or a negative code. */
+long phx_server_submit_screen(PhxServer *srv, const PhTarget *stars, int ns);
+long phx_server_submit_pair(PhxServer *srv, const PhTarget *target, const int *cluster_idx, int n, int n_clones,
+                            double dt_myr);
+/* 0 = done, 1 = still pending after timeout_s (< 0: wait forever), < 0 = the request failed. */
+int phx_server_wait(PhxServer *srv, long id, double timeout_s);
+/* Rows of a finished screening request (star = index within the request), ordered by (star, cluster). */
+int phx_server_screen_rows(PhxServer *srv, long id, PhxRow *rows, long cap, long *n);
+/* PhResult per requested cluster, in request order. */
+int phx_server_pair_results(PhxServer *srv, long id, PhResult *res, int cap);
+/* Arrival, start and completion times (s, monotonic clock) and the number of requests in its batch;
+ * returns the request's status (0 queued, 1 running, 2 done, < 0 failed). */
+int phx_server_info(PhxServer *srv, long id, double t[3], int *batch_reqs);
+int phx_server_release(PhxServer *srv, long id);
 
 /* Host helpers with the reference's semantics. */
 uint64_t phx_seed(const char *a, const char *b); /* reference: src/phygas.c:452-458 */

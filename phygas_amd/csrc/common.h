@@ -69,6 +69,13 @@ int launch_finish_sel_plain(const double *d_trk, const double *d_csoa, const int
 int launch_rows(const long *d_pair_idx, const long *d_ridx, const long *d_rcount, long rcap, int nc,
                 const int *d_sgroup, const double *d_chi2, const int *d_kb, const double *d_sb, const double *d_fout,
                 PhxRow *d_rows, hipStream_t st);
+// Clone draws on the device, one wavefront per pair, in the reference's sequential order; pair p writes its
+// m[p] clones at ps[off[p] ..].
+int launch_pair_draw(const PhTarget *d_t, const PhCluster *d_c, const int *d_ti, const int *d_ci,
+                     const uint64_t *d_seed, const long *d_off, int npairs, int n_clones, const Pot &P, double *d_ps,
+                     int *d_m, hipStream_t st);
+int launch_pair_steps(const long *d_off, const int *d_m, const int *d_psteps, const double *d_pdt, int npairs,
+                      int n_clones, int *d_steps, double *d_dts, hipStream_t st);
 // Quantiles and fractions of each pair's m clones (res[n][3]: best kpc, tbest Myr, vbest kpc/Myr).
 int launch_pair_reduce(const double *d_res, const long *d_off, const int *d_m, const double *d_r50_pc,
                        const double *d_rtot_pc, int npairs, PhResult *d_out, hipStream_t st);

@@ -77,6 +77,7 @@ struct PhxCtx {
     const char *stage_name[MAX_STAGES] = {};
     // pairs
     DBuf pps, psteps, pdts, pst, pres, poff, pm, pr50, prt, pout;
+    DBuf ptg, pcl, pti, pci, pseed, ppst, ppdt;   // device-side draws
 };
 
 static uint64_t fnv_str(uint64_t h, const char *s) {
@@ -124,7 +125,7 @@ extern "C" void phx_destroy(PhxCtx *c) {
                     &c->sr, &c->sseed, &c->sprefix, &c->sgroup, &c->ps0, &c->sig, &c->trk, &c->chi2,
                     &c->kb, &c->sb, &c->seltmp, &c->idx, &c->counts, &c->fout, &c->ridx, &c->rows,
                     &c->pps, &c->psteps, &c->pdts, &c->pst, &c->pres, &c->poff, &c->pm, &c->pr50,
-                    &c->prt, &c->pout};
+                    &c->prt, &c->pout, &c->ptg, &c->pcl, &c->pti, &c->pci, &c->pseed, &c->ppst, &c->ppdt};
     for (DBuf *b : bufs) b->release();
     fast_release(c->fast, c->fw);
     for (int i = 0; i <= MAX_STAGES; i++)
@@ -489,33 +490,118 @@ extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int 
     return 0;
 }
 
+// Clone draws on the device (launch_pair_draw): pair p owns n_clones slots at rank(p) * n_clones, pairs
+// ranked longest trace-back first; unused slots run 0 steps.
+static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
+                             const int *ci, int n_clones, double dt_myr, PhResult *res) {
+    int nt = 0, ncl = 0;
+    for (int p = 0; p < npairs; p++) {
+        nt = std::max(nt, ti[p] + 1);
+        ncl = std::max(ncl, ci[p] + 1);
+    }
+    std::vector<uint64_t> seeds(npairs);
+    std::vector<int> pst(npairs), order(npairs);
+    std::vector<double> pdt(npairs), r50(npairs), rt(npairs);
+    for (int p = 0; p < npairs; p++) {
+        const PhTarget &T = t[ti[p]];
+        const PhCluster &C = cl[ci[p]];
+        seeds[p] = phx_seed(T.id, C.name);
+        pst[p] = (int)ceil(C.tmax_myr / dt_myr);   // reference: src/phygas.c:185-187
+        pdt[p] = -C.tmax_myr / pst[p];
+        r50[p] = C.r50_pc;
+        rt[p] = C.rtot_pc;
+        order[p] = p;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pst[a] > pst[b]; });
+    std::vector<long> off(npairs);
+    for (int r = 0; r < npairs; r++) off[order[r]] = (long)r * n_clones;
+    const size_t n = (size_t)npairs * n_clones;
+    if (c->pps.ensure(96 * n) || c->psteps.ensure(4 * n) || c->pdts.ensure(8 * n) || c->pst.ensure(168 * n) ||
+        c->pres.ensure(24 * n) || c->poff.ensure(8 * (size_t)npairs) || c->pm.ensure(4 * (size_t)npairs) ||
+        c->pr50.ensure(8 * (size_t)npairs) || c->prt.ensure(8 * (size_t)npairs) ||
+        c->pout.ensure(sizeof(PhResult) * (size_t)npairs) || c->ptg.ensure(sizeof(PhTarget) * nt) ||
+        c->pcl.ensure(sizeof(PhCluster) * ncl) || c->pti.ensure(4 * (size_t)npairs) || c->pci.ensure(4 * (size_t)npairs) ||
+        c->pseed.ensure(8 * (size_t)npairs) || c->ppst.ensure(4 * (size_t)npairs) || c->ppdt.ensure(8 * (size_t)npairs))
+        return 1;
+    PHX_CK(hipMemcpyAsync(c->ptg.p, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pcl.p, cl, sizeof(PhCluster) * ncl, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pti.p, ti, 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pci.p, ci, 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pseed.p, seeds.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->ppst.p, pst.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->ppdt.p, pdt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->poff.p, off.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->pr50.p, r50.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->prt.p, rt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemsetAsync(c->pps.p, 0, 96 * n, c->st));
+    stage_begin(c);
+    if (launch_pair_draw(c->ptg.as<PhTarget>(), c->pcl.as<PhCluster>(), c->pti.as<int>(), c->pci.as<int>(),
+                         c->pseed.as<uint64_t>(), c->poff.as<long>(), npairs, n_clones, c->P, c->pps.as<double>(),
+                         c->pm.as<int>(), c->st) ||
+        launch_pair_steps(c->poff.as<long>(), c->pm.as<int>(), c->ppst.as<int>(), c->ppdt.as<double>(), npairs,
+                          n_clones, c->psteps.as<int>(), c->pdts.as<double>(), c->st))
+        return 1;
+    stage_mark(c, "draws");
+    const int smax = *std::max_element(pst.begin(), pst.end());
+    if (trace_clones(c, form, (int)n, smax)) return 1;
+    stage_mark(c, "clones");
+    if (launch_pair_reduce(c->pres.as<double>(), c->poff.as<long>(), c->pm.as<int>(), c->pr50.as<double>(),
+                           c->prt.as<double>(), npairs, c->pout.as<PhResult>(), c->st))
+        return 1;
+    stage_mark(c, "reduce");
+    PHX_CK(hipMemcpyAsync(res, c->pout.p, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_pair_ex(PhxCtx *c, int form, int flags, const PhTarget *t, const PhCluster *cl, int npairs,
+                           const int *ti, const int *ci, int n_clones, double dt_myr, PhResult *res) {
+    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
+    PHX_REQUIRE(n_clones <= 4096, "at most 4096 clones per pair (device reduction size)");
+    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST, "unknown form");
+    PHX_CK(hipSetDevice(c->device));
+    if (npairs == 0) return 0;
+    if (flags & PHX_DRAW_ON_DEVICE) return pair_device_draws(c, form, t, cl, npairs, ti, ci, n_clones, dt_myr, res);
+    return phx_pair(c, form, t, cl, npairs, ti, ci, n_clones, dt_myr, res);
+}
+
 extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
                         const int *ci, int n_clones, double dt_myr, PhResult *res) {
     PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
     PHX_REQUIRE(n_clones <= 4096, "at most 4096 clones per pair (device reduction size)");
+    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST, "unknown form");
     PHX_CK(hipSetDevice(c->device));
     if (npairs == 0) return 0;
-    std::vector<double> ps((size_t)npairs * n_clones * 12);
-    std::vector<int> steps((size_t)npairs * n_clones), m(npairs);
-    std::vector<double> dts((size_t)npairs * n_clones), r50(npairs), rt(npairs);
-    std::vector<long> off(npairs);
-    long n = 0;
-    int smax = 0;
+    // draws per pair (host, reference RNG), then the device layout: pairs longest trace-back first so
+    // every wavefront's clones run the same number of steps; results stay in the caller's pair order
+    std::vector<double> draw((size_t)npairs * n_clones * 12);
+    std::vector<int> m(npairs), pst(npairs), order(npairs);
+    std::vector<double> pdt(npairs), r50(npairs), rt(npairs);
     for (int p = 0; p < npairs; p++) {
         const PhTarget &T = t[ti[p]];
         const PhCluster &C = cl[ci[p]];
-        int st;
-        double dt;
-        off[p] = n;
-        m[p] = phx_pair_draw(&T, &C, n_clones, dt_myr, phx_seed(T.id, C.name), ps.data() + (size_t)n * 12, &st, &dt);
-        for (int k = 0; k < m[p]; k++) {
-            steps[n + k] = st;
-            dts[n + k] = dt;
-        }
-        smax = std::max(smax, st);
-        n += m[p];
+        m[p] = phx_pair_draw(&T, &C, n_clones, dt_myr, phx_seed(T.id, C.name), draw.data() + (size_t)p * n_clones * 12,
+                             &pst[p], &pdt[p]);
         r50[p] = C.r50_pc;
         rt[p] = C.rtot_pc;
+        order[p] = p;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pst[a] > pst[b]; });
+    std::vector<double> ps((size_t)npairs * n_clones * 12);
+    std::vector<int> steps((size_t)npairs * n_clones);
+    std::vector<double> dts((size_t)npairs * n_clones);
+    std::vector<long> off(npairs);
+    long n = 0;
+    int smax = 0;
+    for (int p : order) {
+        off[p] = n;
+        memcpy(ps.data() + (size_t)n * 12, draw.data() + (size_t)p * n_clones * 12, sizeof(double) * 12 * m[p]);
+        for (int k = 0; k < m[p]; k++) {
+            steps[n + k] = pst[p];
+            dts[n + k] = pdt[p];
+        }
+        smax = std::max(smax, pst[p]);
+        n += m[p];
     }
     if (c->pps.ensure(96 * (size_t)std::max(n, 1L)) || c->psteps.ensure(4 * (size_t)std::max(n, 1L)) ||
         c->pdts.ensure(8 * (size_t)std::max(n, 1L)) || c->pst.ensure(168 * (size_t)std::max(n, 1L)) ||

@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libphygas_hip.so")
 
 PLAIN, FAST, FAST64 = 0, 1, 2
+DRAW_ON_DEVICE = 1
 
 ROW_DTYPE = np.dtype([("star", "<i4"), ("cluster", "<i4"), ("kb", "<i4"), ("flags", "<i4"), ("chi2", "<f8"),
                       ("sb", "<f8"), ("t", "<f8"), ("d_nom_pc", "<f8"), ("sig_pc", "<f8"), ("p_r50", "<f8"),
@@ -67,6 +68,7 @@ def lib() -> C.CDLL:
         "phx_seed": (C.c_uint64, [C.c_char_p, C.c_char_p]),
         "phx_pair_draw": (_I, [_P, _P, _I, _D, C.c_uint64, _P, _P, _P]),
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
+        "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -231,12 +233,14 @@ class Engine:
                "phx_pair_clones")
         return b, t, v
 
-    def pair(self, targets, clusters, ti, ci, n_clones: int, dt_myr: float = 0.05, form: int = PLAIN) -> np.ndarray:
-        targets = np.ascontiguousarray(targets, dtype=TARGET_DTYPE)
-        clusters = np.ascontiguousarray(clusters, dtype=CLUSTER_DTYPE)
+    def pair(self, targets, clusters, ti, ci, n_clones: int, dt_myr: float = 0.05, form: int = PLAIN,
+             device_draws: bool = False) -> np.ndarray:
+        """ph_pair for every (targets[ti[i]], clusters[ci[i]]) (reference: src/phygas.c:240-249)."""
+        targets = np.ascontiguousarray(np.atleast_1d(targets), dtype=TARGET_DTYPE)
+        clusters = np.ascontiguousarray(np.atleast_1d(clusters), dtype=CLUSTER_DTYPE)
         ti = np.ascontiguousarray(ti, dtype=np.int32)
         ci = np.ascontiguousarray(ci, dtype=np.int32)
         res = np.zeros(len(ti), RESULT_DTYPE)
-        _check(self._L.phx_pair(self.h, form, _p(targets), _p(clusters), len(ti), _p(ti), _p(ci), n_clones, dt_myr,
-                                _p(res)), "phx_pair")
+        _check(self._L.phx_pair_ex(self.h, form, DRAW_ON_DEVICE if device_draws else 0, _p(targets), _p(clusters),
+                                   len(ti), _p(ti), _p(ci), n_clones, dt_myr, _p(res)), "phx_pair_ex")
         return res

@@ -149,3 +149,62 @@ def test_fast_vs_plain_on_work_unit_sample(unit):
     _report("fast_vs_plain_chi2_rel_max", c2.max())
     assert np.mean(dp > 1e-4) <= 1e-3
     assert c2.max() < 1e-4
+
+
+@pytest.fixture(scope="module")
+def pulsar_unit(workload):
+    """1 synthetic pulsar x 300 clusters x 200 clones, dt 0.05 Myr (clones drawn on the host, seed per pair)."""
+    return workload.make_pulsars(1, 1), workload.make_clusters(300, 1)
+
+
+@pytest.fixture(scope="module")
+def pulsar_oracle(pulsar_unit, oracle):
+    t, cl = pulsar_unit
+    ti, ci = np.zeros(len(cl), np.int32), np.arange(len(cl), dtype=np.int32)
+    return ti, ci, oracle.pair_run(t, cl, ti, ci, n_clones=200, dt_myr=0.05, nthreads=16)
+
+
+@pytest.mark.parametrize("form", ["plain", "fast"])
+def test_pair_forms_match_oracle(pulsar_unit, pulsar_oracle, form):
+    """Monte Carlo quantiles within 1e-3 relative and identical clone counts; the plain form
+    within 1e-9.  Fractions may differ only by clones within rounding of r50 / rtot."""
+    t, cl = pulsar_unit
+    ti, ci, want = pulsar_oracle
+    with hip.Engine(0) as e:
+        got = e.pair(t, cl, ti, ci, n_clones=200, dt_myr=0.05, form=hip.PLAIN if form == "plain" else hip.FAST)
+    assert np.array_equal(got["n"], want["n"])
+    tol = 1e-9 if form == "plain" else 1e-3
+    worst = 0.0
+    for f in ("d05", "d50", "d95", "t05", "t50", "t95", "v05", "v50", "v95"):
+        rel = np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-6)
+        worst = max(worst, rel.max())
+        assert rel.max() < (tol if not f.startswith("t") else max(tol, 1e-6)), (f, rel.max())
+    _report(f"pair_{form}_quantile_rel_max", worst)
+    dfr = np.abs(got["f_r50"] - want["f_r50"]) * 200
+    dft = np.abs(got["f_rtot"] - want["f_rtot"]) * 200
+    _report(f"pair_{form}_fraction_clone_flips", float(dfr.sum() + dft.sum()))
+    assert dfr.max() <= (0 if form == "plain" else 1) and dft.max() <= (0 if form == "plain" else 1)
+
+
+def test_device_draws_match_host_reference(workload, oracle):
+    """Clones drawn on the device (speculative prefix over the splitmix64 counter) give the reference's clone
+    counts and, with the plain FP64 tracer, the same quantiles to 1e-9; the second target's parallax draws
+    fail and retry often, exercising the irregular path."""
+    tg = np.concatenate([workload.make_pulsars(1, 1), workload.make_stars(2, 7)])
+    tg[2]["plx"], tg[2]["e_plx"] = 0.15, 0.3
+    cl = workload.make_clusters(40, 2)
+    ti = np.repeat(np.arange(3, dtype=np.int32), 40)
+    ci = np.tile(np.arange(40, dtype=np.int32), 3)
+    want = oracle.pair_run(tg, cl, ti, ci, n_clones=150, dt_myr=0.1, nthreads=16)
+    with hip.Engine(0) as e:
+        got = e.pair(tg, cl, ti, ci, n_clones=150, dt_myr=0.1, form=hip.PLAIN, device_draws=True)
+        fast = e.pair(tg, cl, ti, ci, n_clones=150, dt_myr=0.1, form=hip.FAST, device_draws=True)
+    assert np.array_equal(got["n"], want["n"]) and np.array_equal(fast["n"], want["n"])
+    worst = 0.0
+    for f in ("d05", "d50", "d95", "v05", "v50", "v95"):
+        rel = np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-6)
+        worst = max(worst, rel.max())
+        assert rel.max() < 1e-9, f
+        assert (np.abs(fast[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-6)).max() < 1e-3, f
+    _report("device_draws_quantile_rel_max", worst)
+    assert np.array_equal(got["f_r50"], want["f_r50"])

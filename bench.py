@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--serve-rate", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the screening shards by star with no "
@@ -152,6 +154,20 @@ def main() -> int:
                 "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
                 "traffic": None, "work_per_launch_flop": flops, "avg_launch_ms": k_ms}
 
+    e.close()
+    serving = None
+    if args.serve_requests > 0:
+        from phygas_amd import serve
+        mix = serve.request_mix(args.serve_requests, args.serve_rate, seed=5, n_clusters=len(clusters))
+        serving = {"rate_hz": args.serve_rate, "requests": args.serve_requests,
+                   "mix": "70% screening chunks of 256..20,000 stars (log-uniform), 30% pulsar jobs of 100..600 "
+                          "clusters x 1,000 clones; Poisson arrivals, seed 5"}
+        for name, f in (("fast", hip.FAST), ("plain", hip.PLAIN)):
+            if f == hip.FAST and form != hip.FAST:
+                continue
+            with serve.Server(clusters, form=f, screen_slots=2, pair_slots=1, p_min=p_min) as srv:
+                serving[name] = serve.run_mix(srv, mix)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -172,8 +188,9 @@ def main() -> int:
     if plain:
         line["plain"] = plain
         line["speedup_vs_plain"] = plain["ms"] / ms
-    print(json.dumps(line), flush=True)
-    e.close()
+    if serving:
+        line["serving"] = serving
+    print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
     return 0
 
 

@@ -93,6 +93,10 @@ int phx_kept(PhxCtx *ctx, long *n);
  * into names[] (static strings). */
 int phx_stage_times(PhxCtx *ctx, double *ms, const char **names, int cap);
 void phx_set_timing(PhxCtx *ctx, int on);
+/* Fast-form passes captured into a hipGraph and replayed while batch sizes, parameters and buffers stay the
+ * same (a pass with timing on runs eagerly, its stage events between the kernels). */
+void phx_set_graphs(PhxCtx *ctx, int on);
+int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
 
 /* Parity access to intermediate device results of the last plain pass / the cluster tables. */
 int phx_cluster_tracks(PhxCtx *ctx, int *n, double *pos, double *vel, double *S, int npts_cap);

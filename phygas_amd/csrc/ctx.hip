@@ -143,6 +143,14 @@ extern "C" void *phx_stream(PhxCtx *c) { return (void *)c->st; }
 
 extern "C" void phx_set_timing(PhxCtx *c, int on) { c->timing = on != 0; }
 
+extern "C" void phx_set_graphs(PhxCtx *c, int on) { c->fw.use_graphs = on != 0; }
+
+extern "C" int phx_graph_stats(PhxCtx *c, long *captures, long *replays) {
+    *captures = c->fw.captures;
+    *replays = c->fw.replays;
+    return 0;
+}
+
 static void stage_begin(PhxCtx *c) {
     c->nstage = 0;
     if (c->timing) (void)hipEventRecord(c->ev[0], c->st);
@@ -337,7 +345,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     a.group = c->sgroup.as<int>(); a.cl = c->cl.as<PhCluster>(); a.csoa = c->csoa.as<double>();
     a.cn = c->cn.as<int>(); a.R2 = c->R2.as<double>(); a.r50 = c->r50.as<double>(); a.rt = c->rt.as<double>();
     auto mark = [](void *ctx, const char *name) { stage_mark((PhxCtx *)ctx, name); };
-    return fast_screen(c->fast, c->fw, a, c->st, mark, c);
+    return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
 }
 
 extern "C" int phx_screen(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min) {

@@ -22,6 +22,11 @@ struct FastWork {
     long cand_cap = 0, kept_cap = 0, row_cap = 0;
     long last_cand = 0;
     int scan_blocks = 0;               // grid of the persistent tile scan (0: default)
+    // hipGraph of the pass's launches, replayed while sizes, parameters and buffers stay the same
+    bool use_graphs = false;
+    void *graph = nullptr, *gexec = nullptr;
+    uint64_t gkey = 0;
+    long captures = 0, replays = 0;
 };
 
 struct FastArgs {

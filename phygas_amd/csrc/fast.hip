@@ -602,7 +602,7 @@ static int build_cluster_tiles(FastTables &T, double chi2_max, hipStream_t st) {
     return 0;
 }
 
-int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
+static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st) {
     const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     const long npairs = (long)ns * a.nc;
     if (W.cand_cap == 0) W.cand_cap = std::max(1L << 20, npairs / 5);
@@ -617,7 +617,12 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
         ensure(W.mem[W_ROWS], W.bytes[W_ROWS], sizeof(PhxRow) * (size_t)W.row_cap) ||
         ensure(W.mem[W_CNT], W.bytes[W_CNT], 4 * sizeof(unsigned long long)))
         return 1;
-    if (build_cluster_tiles(T, a.chi2_max, st)) return 1;
+    return build_cluster_tiles(T, a.chi2_max, st);
+}
+
+// The pass's launches only (no allocation, no host synchronisation): capturable into a hipGraph.
+static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
+    const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     unsigned long long *cnt = (unsigned long long *)W.mem[W_CNT];
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cnt, 4);
     PHX_CK(hipGetLastError());
@@ -625,7 +630,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     float4 *trkf = (float4 *)W.mem[W_TRKF];
     float4 *stlo = (float4 *)W.mem[W_STILE], *sthi = stlo + (size_t)ns * ntl;
     if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) return 1;
-    mark(mark_ctx, "draws");
+    if (mark) mark(mark_ctx, "draws");
     if (a.exact_tracks) {   // the plain FP64 tracks: rows identical to the plain form's
         if (launch_lin_plain(ps0, sig, nullptr, trk, a.P, PH_LIN_EPS_POS, PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB,
                              PH_LIN_SUB, 0, npts, npts, ns, 0, nullptr, st))
@@ -636,28 +641,72 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
                            make_potf(a.P), npts, ns);
     }
     PHX_CK(hipGetLastError());
-    mark(mark_ctx, "tracks");
+    if (mark) mark(mark_ctx, "tracks");
     hipLaunchKernelGGL(k_star_tiles, dim3(blocks_for((long)ns * ntl, 256)), dim3(256), 0, st, trk, trkf, npts, ntl, ns,
                        (float)sqrt(a.chi2_max), stlo, sthi);
     PHX_CK(hipGetLastError());
-    mark(mark_ctx, "tiles");
+    if (mark) mark(mark_ctx, "tiles");
     hipLaunchKernelGGL(k_scan_tiles, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
                        (const float4 *)T.mem[T_CTLO], (const float4 *)T.mem[T_CTHI], (const int *)T.mem[T_CNT],
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, stlo, sthi, ns, ntl, (Cand *)W.mem[W_CAND], cnt,
                        W.cand_cap);
     PHX_CK(hipGetLastError());
-    mark(mark_ctx, "scan");
+    if (mark) mark(mark_ctx, "scan");
     hipLaunchKernelGGL(k_refine, dim3(4096), dim3(256), 0, st, (const Cand *)W.mem[W_CAND], cnt, W.cand_cap, trk, trkf,
                        (const double *)T.mem[T_CTRK], (const float4 *)T.mem[T_CTRKF], a.cn, a.R2, npts, a.chi2_max,
                        (Kept *)W.mem[W_KEPT], cnt + 1, W.kept_cap);
     PHX_CK(hipGetLastError());
-    mark(mark_ctx, "refine");
+    if (mark) mark(mark_ctx, "refine");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;
     hipLaunchKernelGGL(k_finish_fast, dim3(4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + 1, W.kept_cap,
                        trk, (const double *)T.mem[T_CTRK], a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min,
                        (PhxRow *)W.mem[W_ROWS], cnt + 2, W.row_cap);
     PHX_CK(hipGetLastError());
-    mark(mark_ctx, "finish");
+    if (mark) mark(mark_ctx, "finish");
+    return 0;
+}
+
+static void graph_drop(FastWork &W) {
+    if (W.gexec) (void)hipGraphExecDestroy((hipGraphExec_t)W.gexec);
+    if (W.graph) (void)hipGraphDestroy((hipGraph_t)W.graph);
+    W.gexec = W.graph = nullptr;
+}
+
+int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
+    if (fast_alloc(T, W, a, st)) return 1;
+    if (!W.use_graphs || mark) {   // eager launches (timed stages need their events between kernels)
+        return fast_launch(T, W, a, st, mark, mark_ctx);
+    }
+    // the key of a captured pass: sizes, parameters and every buffer it touches
+    uint64_t key = 1469598103934665603ULL;
+    auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };
+    mix((uint64_t)a.ns); mix((uint64_t)a.nc); mix((uint64_t)a.npts); mix((uint64_t)a.n_samp);
+    mix((uint64_t)a.exact_tracks); mix((uint64_t)W.scan_blocks);
+    double dv[2] = {a.chi2_max, a.p_min};
+    uint64_t bits[2];
+    memcpy(bits, dv, sizeof bits);
+    mix(bits[0]); mix(bits[1]);
+    const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt};
+    for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
+    for (int i = 0; i < 16; i++) { mix((uint64_t)(uintptr_t)W.mem[i]); mix((uint64_t)(uintptr_t)T.mem[i]); }
+    mix((uint64_t)W.cand_cap); mix((uint64_t)W.kept_cap); mix((uint64_t)W.row_cap);
+    if (!W.gexec || W.gkey != key) {
+        graph_drop(W);
+        hipGraph_t g = nullptr;
+        PHX_CK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = fast_launch(T, W, a, st, nullptr, nullptr);
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        PHX_CK(ec);
+        hipGraphExec_t ex = nullptr;
+        PHX_CK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        W.graph = g;
+        W.gexec = ex;
+        W.gkey = key;
+        W.captures++;
+    }
+    PHX_CK(hipGraphLaunch((hipGraphExec_t)W.gexec, st));
+    W.replays++;
     return 0;
 }
 
@@ -693,6 +742,7 @@ int fast_z0_check(uint64_t seed, long n, double *worst, hipStream_t st) {
 }
 
 void fast_release(FastTables &T, FastWork &W) {
+    graph_drop(W);
     for (int i = 0; i < 16; i++) {
         if (T.mem[i]) (void)hipFree(T.mem[i]);
         if (W.mem[i]) (void)hipFree(W.mem[i]);

@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
         "phx_pair_draw": (_I, [_P, _P, _I, _D, C.c_uint64, _P, _P, _P]),
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
+        "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -182,6 +183,14 @@ class Engine:
 
     def set_timing(self, on: bool = True):
         self._L.phx_set_timing(self.h, int(on))
+
+    def set_graphs(self, on: bool = True):
+        self._L.phx_set_graphs(self.h, int(on))
+
+    def graph_stats(self) -> tuple[int, int]:
+        c, r = _L(0), _L(0)
+        self._L.phx_graph_stats(self.h, C.byref(c), C.byref(r))
+        return c.value, r.value
 
     def stage_times(self) -> dict[str, float]:
         ms = np.zeros(16)

@@ -158,6 +158,10 @@ int phx_server_pair_results(PhxServer *srv, long id, PhResult *res, int cap);
 int phx_server_info(PhxServer *srv, long id, double t[3], int *batch_reqs);
 int phx_server_release(PhxServer *srv, long id);
 
+/* Pair-mode prefilter on the device (reference: src/phygas.c:98-168 ph_track_* and ph_prefilter, as
+ * src/phygas_cli.c:103-117 runs them): mask[j * nc + i] = 1 when target j may have reached cluster i. */
+int phx_prefilter(PhxCtx *ctx, const PhTarget *t, int nt, const PhCluster *c, int nc, unsigned char *mask);
+
 /* Host helpers with the reference's semantics. */
 uint64_t phx_seed(const char *a, const char *b); /* reference: src/phygas.c:452-458 */
 int phx_pair_draw(const PhTarget *t, const PhCluster *c, int n_clones, double dt_myr, uint64_t seed, double *ps,

@@ -69,6 +69,11 @@ int launch_finish_sel_plain(const double *d_trk, const double *d_csoa, const int
 int launch_rows(const long *d_pair_idx, const long *d_ridx, const long *d_rcount, long rcap, int nc,
                 const int *d_sgroup, const double *d_chi2, const int *d_kb, const double *d_sb, const double *d_fout,
                 PhxRow *d_rows, hipStream_t st);
+// Pair-mode prefilter of nt targets x nc clusters: nominal paths with 32-draw spreads (seeds "track"|name),
+// mask[j * nc + i] = 1 when target j may reach cluster i.  Paths [obj][npts][3].
+int launch_prefilter(const PhTarget *d_t, const uint64_t *d_tseed, const int *d_tn, int nt, const PhCluster *d_c,
+                     const uint64_t *d_cseed, const int *d_cn, int nc, int npts, const Pot &P, double *d_tpath,
+                     double *d_tspr, double *d_cpath, double *d_cspr, unsigned char *d_mask, hipStream_t st);
 // Clone draws on the device, one wavefront per pair, in the reference's sequential order; pair p writes its
 // m[p] clones at ps[off[p] ..].
 int launch_pair_draw(const PhTarget *d_t, const PhCluster *d_c, const int *d_ti, const int *d_ci,

@@ -498,6 +498,47 @@ extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int 
     return 0;
 }
 
+// Pair-mode prefilter on the device (reference: src/phygas_cli.c:103-117, src/phygas.c:98-168): target paths
+// over the longest cluster trace-back, cluster paths over their own; mask[j * nc + i].
+extern "C" int phx_prefilter(PhxCtx *c, const PhTarget *t, int nt, const PhCluster *cl, int nc, unsigned char *mask) {
+    PHX_REQUIRE(nt >= 0 && nc >= 0, "negative size");
+    PHX_CK(hipSetDevice(c->device));
+    if (nt == 0 || nc == 0) return 0;
+    double tmax = 0;
+    for (int i = 0; i < nc; i++) tmax = std::max(tmax, cl[i].tmax_myr);
+    const int npts = (int)ceil(tmax / STEP_MYR) + 1;
+    std::vector<uint64_t> ts(nt), cs(nc);
+    std::vector<int> tn(nt, npts), cn(nc);
+    for (int j = 0; j < nt; j++) ts[j] = phx_seed("track", t[j].id);
+    for (int i = 0; i < nc; i++) {
+        cs[i] = phx_seed("track", cl[i].name);
+        cn[i] = (int)ceil(cl[i].tmax_myr / STEP_MYR) + 1;
+    }
+    DBuf dt, dc, dts, dcs, dtn, dcn, tp, tsp, cp, csp, dm;
+    int rc = dt.ensure(sizeof(PhTarget) * nt) || dc.ensure(sizeof(PhCluster) * nc) || dts.ensure(8 * (size_t)nt) ||
+             dcs.ensure(8 * (size_t)nc) || dtn.ensure(4 * (size_t)nt) || dcn.ensure(4 * (size_t)nc) ||
+             tp.ensure(24 * (size_t)nt * npts) || tsp.ensure(16 * (size_t)nt) || cp.ensure(24 * (size_t)nc * npts) ||
+             csp.ensure(16 * (size_t)nc) || dm.ensure((size_t)nt * nc);
+    if (!rc) {
+        PHX_CK(hipMemcpyAsync(dt.p, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
+        PHX_CK(hipMemcpyAsync(dc.p, cl, sizeof(PhCluster) * nc, hipMemcpyHostToDevice, c->st));
+        PHX_CK(hipMemcpyAsync(dts.p, ts.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, c->st));
+        PHX_CK(hipMemcpyAsync(dcs.p, cs.data(), 8 * (size_t)nc, hipMemcpyHostToDevice, c->st));
+        PHX_CK(hipMemcpyAsync(dtn.p, tn.data(), 4 * (size_t)nt, hipMemcpyHostToDevice, c->st));
+        PHX_CK(hipMemcpyAsync(dcn.p, cn.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, c->st));
+        rc = launch_prefilter(dt.as<PhTarget>(), dts.as<uint64_t>(), dtn.as<int>(), nt, dc.as<PhCluster>(),
+                              dcs.as<uint64_t>(), dcn.as<int>(), nc, npts, c->P, tp.as<double>(), tsp.as<double>(),
+                              cp.as<double>(), csp.as<double>(), dm.as<unsigned char>(), c->st);
+        if (!rc) {
+            PHX_CK(hipMemcpyAsync(mask, dm.p, (size_t)nt * nc, hipMemcpyDeviceToHost, c->st));
+            PHX_CK(hipStreamSynchronize(c->st));
+        }
+    }
+    DBuf *all[] = {&dt, &dc, &dts, &dcs, &dtn, &dcn, &tp, &tsp, &cp, &csp, &dm};
+    for (DBuf *b : all) b->release();
+    return rc;
+}
+
 // Clone draws on the device (launch_pair_draw): pair p owns n_clones slots at rank(p) * n_clones, pairs
 // ranked longest trace-back first; unused slots run 0 steps.
 static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,

@@ -362,6 +362,84 @@ int launch_rows(const long *d_pair_idx, const long *d_ridx, const long *d_rcount
     return 0;
 }
 
+// ---------------------------------------------------------------- pair-mode prefilter
+// Nominal path (0.5-Myr points, 10 substeps each) and the RMS spread of n_spread draws about the nominal
+// (reference: src/phygas.c:98-144), one lane per object; paths stored [obj][npts][3].
+template <class Rec, int IsCluster>
+__global__ __launch_bounds__(128) void k_track(const Rec *__restrict__ recs, const uint64_t *__restrict__ seeds,
+                                               const int *__restrict__ nobj, int n, int n_spread, int npts, Pot P,
+                                               double *__restrict__ path, double *__restrict__ spread) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Rec r = recs[i];
+    V3 np = v3(0, 0, 0), nv = v3(0, 0, 0);
+    if (IsCluster) sample_cluster(P, (const PhCluster &)r, nullptr, np, nv);
+    else sample_target(P, (const PhTarget &)r, nullptr, np, nv);
+    uint64_t st = seeds[i];
+    double sp = 0, sv = 0;
+    int k = 0;
+    for (int d = 0; d < n_spread; d++) {
+        V3 p, v;
+        const int rc = IsCluster ? sample_cluster(P, (const PhCluster &)r, &st, p, v)
+                                 : sample_target(P, (const PhTarget &)r, &st, p, v);
+        if (rc != 0) continue;
+        const V3 dp = p - np, dv = v - nv;
+        sp += dot(dp, dp);
+        sv += dot(dv, dv);
+        k++;
+    }
+    spread[2 * i] = k ? sqrt(sp / k) : 0;
+    spread[2 * i + 1] = k ? sqrt(sv / k) : 0;
+    Body b = body_start(P, np, nv);
+    const int n_i = nobj[i];
+    double *o = path + (size_t)i * npts * 3;
+    o[0] = b.pos.x; o[1] = b.pos.y; o[2] = b.pos.z;
+    const double dt = -STEP_MYR / 10;
+    for (int q = 1; q < n_i; q++) {
+        for (int s = 0; s < 10; s++) kdk(P, b, dt);
+        o[q * 3] = b.pos.x; o[q * 3 + 1] = b.pos.y; o[q * 3 + 2] = b.pos.z;
+    }
+}
+
+// One lane per (target, cluster): can the target have come within reach (reference: src/phygas.c:158-168)?
+__global__ __launch_bounds__(256) void k_prefilter(const double *__restrict__ tpath, const double *__restrict__ tspr,
+                                                   const int *__restrict__ tn, const double *__restrict__ cpath,
+                                                   const double *__restrict__ cspr, const int *__restrict__ cnn,
+                                                   const PhCluster *__restrict__ cl, int nt, int nc, int npts,
+                                                   unsigned char *__restrict__ mask) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nt * nc) return;
+    const int j = (int)(id / nc), i = (int)(id % nc);
+    const int n = min(tn[j], cnn[i]);
+    const double sp = hypot(tspr[2 * j], cspr[2 * i]), sv = hypot(tspr[2 * j + 1], cspr[2 * i + 1]);
+    const double r0 = (3.0 * cl[i].rtot_pc + 10.0) / 1000.0;
+    const double *a = tpath + (size_t)j * npts * 3, *b = cpath + (size_t)i * npts * 3;
+    unsigned char hit = 0;
+    for (int k = 0; k + 1 < n && !hit; k++) {
+        const V3 m0 = v3(a[k * 3] - b[k * 3], a[k * 3 + 1] - b[k * 3 + 1], a[k * 3 + 2] - b[k * 3 + 2]);
+        const V3 m1 = v3(a[k * 3 + 3] - b[k * 3 + 3], a[k * 3 + 4] - b[k * 3 + 4], a[k * 3 + 5] - b[k * 3 + 5]);
+        const double s = seg_param(m0, m1);
+        const double d = norm(m0 + (m1 - m0) * s);
+        const double t = (k + s) * STEP_MYR;
+        if (d < r0 + 3.0 * (sp + sv * t)) hit = 1;
+    }
+    mask[id] = hit;
+}
+
+int launch_prefilter(const PhTarget *d_t, const uint64_t *d_tseed, const int *d_tn, int nt, const PhCluster *d_c,
+                     const uint64_t *d_cseed, const int *d_cn, int nc, int npts, const Pot &P, double *d_tpath,
+                     double *d_tspr, double *d_cpath, double *d_cspr, unsigned char *d_mask, hipStream_t st) {
+    if (nt <= 0 || nc <= 0) return 0;
+    hipLaunchKernelGGL((k_track<PhTarget, 0>), dim3(blocks_for(nt, 128)), dim3(128), 0, st, d_t, d_tseed, d_tn, nt, 32,
+                       npts, P, d_tpath, d_tspr);
+    hipLaunchKernelGGL((k_track<PhCluster, 1>), dim3(blocks_for(nc, 128)), dim3(128), 0, st, d_c, d_cseed, d_cn, nc,
+                       32, npts, P, d_cpath, d_cspr);
+    hipLaunchKernelGGL(k_prefilter, dim3(blocks_for((long)nt * nc, 256)), dim3(256), 0, st, d_tpath, d_tspr, d_tn,
+                       d_cpath, d_cspr, d_cn, d_c, nt, nc, npts, d_mask);
+    PHX_CK(hipGetLastError());
+    return 0;
+}
+
 // ---------------------------------------------------------------- Monte Carlo draws on the device
 // The clones of a pair are drawn from one splitmix64 stream in clone order (reference: src/phygas.c:182-198):
 // target draw then cluster draw, each retried while the parallax / distance is not positive, a clone that

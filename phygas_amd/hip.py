@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
+        "phx_prefilter": (_I, [_P, _P, _I, _P, _I, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -232,6 +233,15 @@ class Engine:
         return float(w[0])
 
     # ---- Monte Carlo pairs
+    def prefilter(self, targets, clusters) -> np.ndarray:
+        """ph_prefilter of every (target, cluster) as phygas_cli runs it (reference: src/phygas_cli.c:103-117)."""
+        targets = np.ascontiguousarray(np.atleast_1d(targets), dtype=TARGET_DTYPE)
+        clusters = np.ascontiguousarray(np.atleast_1d(clusters), dtype=CLUSTER_DTYPE)
+        mask = np.zeros((len(targets), len(clusters)), np.uint8)
+        _check(self._L.phx_prefilter(self.h, _p(targets), len(targets), _p(clusters), len(clusters), _p(mask)),
+               "phx_prefilter")
+        return mask.astype(bool)
+
     def pair_clones(self, ps, steps, dts, form: int = PLAIN):
         ps = np.ascontiguousarray(ps, dtype=np.float64)
         n = len(ps)

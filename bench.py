@@ -62,6 +62,7 @@ def cpu_baseline(clusters, stars, threads: int) -> dict | None:
 def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float, p_min: float):
     for _ in range(warmup):
         e.screen_async(form, chi2_max, 64, p_min)
+        e.kept()          # sizes the selection buffers (a pass whose buffers overflowed is run again)
     e.sync()
     e.set_timing(True)
     stage_sum: dict[str, float] = {}
@@ -70,6 +71,7 @@ def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float,
         e.screen_async(form, chi2_max, 64, p_min)
         for k, v in e.stage_times().items():     # waits for the pass; events are on the engine's stream
             stage_sum[k] = stage_sum.get(k, 0.0) + v
+        e.kept()          # every timed pass is complete: an overflowed pass is run again inside the timing
     e.sync()
     t1 = time.perf_counter()
     e.set_timing(False)
@@ -122,6 +124,7 @@ def main() -> int:
             raise
     rows = e.rows()
     kept = e.kept()
+    candidates = e.candidates()
     result = {"form": "fast" if form == hip.FAST else "plain", "ms": ms, "stages": stages, "rows": len(rows),
               "kept": kept}
 
@@ -182,7 +185,7 @@ def main() -> int:
                    len(control), "clusters": len(clusters), "pairs_per_step": pairs, "chi2_max": chi2_max,
                    "n_samp": 64, "r_kernel": 2.0, "p_min": p_min, "form": result["form"],
                    "cluster_tables_s": tables_s},
-        "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept,
+        "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "candidate_pairs": candidates,
         "roofline": roof, "cpu_baseline": cpu,
     }
     if plain:

@@ -89,6 +89,9 @@ int phx_screen(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min)
 int phx_rows(PhxCtx *ctx, PhxRow *rows, long cap, long *n);
 /* Pairs that passed the chi2 cut in the last pass (finished pairs). */
 int phx_kept(PhxCtx *ctx, long *n);
+/* Pairs the last pass evaluated point by point: every pair for the plain form, the pairs left by the tile
+ * scan for the fast forms. */
+int phx_candidates(PhxCtx *ctx, long *n);
 /* Durations (ms, HIP events on the context's stream) of the last pass's stages; returns the count, names
  * into names[] (static strings). */
 int phx_stage_times(PhxCtx *ctx, double *ms, const char **names, int cap);

@@ -399,6 +399,13 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
     return last_counts(c, n, &r);
 }
 
+extern "C" int phx_candidates(PhxCtx *c, long *n) {
+    long kept, r;
+    if (last_counts(c, &kept, &r)) return 1;
+    *n = c->last_form == PHX_FORM_PLAIN ? (long)c->st_h.size() * c->nc : c->fw.last_cand;
+    return 0;
+}
+
 extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;

@@ -1,0 +1,253 @@
+// Tile culling of the fast screening scan: per-object tile records and the two pair tests.
+//
+// A tile j of an object holds the points k in [8j, 8j+8) of its track (the points whose chi2 the
+// reference evaluates: reference src/phygas.c:395-403 / src/phygas_kernel_body.h:124-163), and the
+// positions [8j, 8j+8] (the far end of the last point's segment).  Everything is expressed in a frame
+// that co-rotates with the Galaxy: point k is rotated by R_k, the rotation by OMEGA_PT * k about the
+// Galactic z axis, so that the circular part of the motion drops out and a tile's positions lie close to
+// a straight line.
+//
+// Record of an object tile (built from its FP64 track):
+//   a, b   least-squares line p'_k ~ a + b (k - tc), tc = 8j + 4, through the rotated positions
+//   e      largest distance of a rotated position from that line (the tube radius)
+//   U      3x3 matrix with U >= R_k S_k R_k^T for every point of the tile (Loewner order): the rotated
+//          covariance of the tile's last point plus eta I, eta >= the largest Gershgorin bound of
+//          R_k S_k R_k^T - R_kl S_kl R_kl^T over the tile; clusters add the pair's kernel R^2 I
+//   lam    upper bound of the largest eigenvalue of U (the smaller of trace and Gershgorin bound)
+//
+// For a pair (star s, cluster c) and a point k of tile j, the reference's value is
+// chi2_k = min(q(m0), q(ms)) with q(x) = x^T M_k^-1 x, M_k = S_s,k + S_c,k + R^2 I, m0 = p_s,k - p_c,k and
+// ms a point of the segment m0 -> m1 = p_s,k+1 - p_c,k+1.  Rotated by R_k, m0 lies within e_s + e_c of the
+// axis da + db t (da = a_s - a_c, db = b_s - b_c, t in [-4, t1]); m1 rotated by R_k instead of R_k+1 moves
+// by at most ROT_W |m1|, and every point of the segment is a convex combination of its ends.  So every x
+// the reference evaluates in the tile lies in the tube  T = {da + db t + r : t in [-4, t1], |r| <= rho},
+// rho = e_s + e_c + ROT_W (|da| + 4 |db| + e_s + e_c), and q(x) >= x^T U^-1 x with U = U_s + U_c.
+//
+//   level 1 (FP32): x^T U^-1 x >= |x|^2 / lam(U) >= (dist(0, axis segment) - rho)^2 / (lam_s + lam_c)
+//   level 2 (FP64): for every vector y, x^T U^-1 x >= 2 y.x - y^T U y (the quadratic form's dual), so
+//                   min over T >= 2 (min_t y.(da + db t) - rho |y|) - y^T U y; y = U^-1 x* with x* the
+//                   U^-1-closest point of the axis segment makes the bound tight.  Evaluated as
+//                   det^2 times it with z = det y = adj(U) x*, and compared with an explicit bound of its
+//                   own rounding error.
+//
+// A tile that fails either test holds no point with chi2 < chi2_max (1 + CUT_REL), so it can hold neither
+// the minimum of a kept pair nor change which pairs are kept.  CUT_REL covers the reference's own FP64
+// rounding of chi2 for matrices with condition number below 1e9 (cond(M) <= trace(M) / R^2); level 2 does
+// not prune pairs beyond that.
+#pragma once
+
+#include "force32.h"
+
+namespace phx {
+
+constexpr int TILE = 8;
+constexpr double OMEGA_PT = -0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122;   // frame rotation per 0.5-Myr point
+constexpr double ROT_W = 0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122 * 1.000001;   // |R(w) x - x| <= ROT_W |x|
+constexpr double CUT_REL = 1e-6;
+
+struct TileRec1 {   // FP32, level 1
+    float ax, ay, az, e, bx, by, bz, lam;
+};
+struct TileRec2 {   // FP64, level 2
+    double a[3], b[3], U[6], e, R2, t1, pad;
+};
+
+// Neighbouring floats (finite inputs): one step towards +inf / -inf.
+PHX_HDF float next_up(float f) {
+    if (f == 0.0f) return 1.4e-45f;
+    union { float f; int i; } u = {f};
+    u.i += f > 0.0f ? 1 : -1;
+    return u.f;
+}
+PHX_HDF float next_down(float f) { return -next_up(-f); }
+PHX_HDF float up(double v) {   // float >= v
+    const float f = (float)v;
+    return (double)f < v ? next_up(f) : f;
+}
+
+// Largest-eigenvalue upper bound of a symmetric PSD 3x3 (xx xy xz yy yz zz): min(trace, Gershgorin).
+PHX_HDF double lam_bound(const double *S) {
+    const double tr = S[0] + S[3] + S[5];
+    const double g0 = S[0] + fabs(S[1]) + fabs(S[2]), g1 = S[3] + fabs(S[1]) + fabs(S[4]),
+                 g2 = S[5] + fabs(S[2]) + fabs(S[4]);
+    const double g = fmax(g0, fmax(g1, g2));
+    return fmax(fmin(tr, g), 0.0);
+}
+
+// R_k S R_k^T of a symmetric 3x3 (xx xy xz yy yz zz), R_k the rotation by OMEGA_PT k about z.
+PHX_HDF void rot_cov(const double *S, double c, double s, double *o) {
+    const double cc = c * c, ss = s * s, cs = c * s;
+    o[0] = cc * S[0] - 2.0 * cs * S[1] + ss * S[3];
+    o[1] = cs * (S[0] - S[3]) + (cc - ss) * S[1];
+    o[2] = c * S[2] - s * S[4];
+    o[3] = ss * S[0] + 2.0 * cs * S[1] + cc * S[3];
+    o[4] = s * S[2] + c * S[4];
+    o[5] = S[5];
+}
+
+// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).
+PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2) {
+    const int k0 = j * TILE;
+    const int kl = k0 + TILE < n - 1 ? k0 + TILE : n - 1;   // last position
+    const int kp = (k0 + TILE < n ? k0 + TILE : n) - 1;     // last point
+    const int m = kl - k0 + 1;
+    const double tc = k0 + 0.5 * TILE;
+    double px[TILE + 1], py[TILE + 1], pz[TILE + 1];
+    double st = 0, stt = 0, sx = 0, sy = 0, sz = 0, stx = 0, sty = 0, stz = 0;
+#pragma unroll
+    for (int i = 0; i <= TILE; i++) {
+        const int k = k0 + (i < m ? i : m - 1);
+        const double *p = trk + (size_t)k * 12;
+        const double th = OMEGA_PT * k, c = cos(th), s = sin(th);
+        px[i] = c * p[0] - s * p[1];
+        py[i] = s * p[0] + c * p[1];
+        pz[i] = p[2];
+        if (i < m) {
+            const double t = k - tc;
+            st += t; stt += t * t;
+            sx += px[i]; sy += py[i]; sz += pz[i];
+            stx += t * px[i]; sty += t * py[i]; stz += t * pz[i];
+        }
+    }
+    double b[3] = {0.0, 0.0, 0.0};
+    const double den = m * stt - st * st;
+    if (m > 1 && den > 0.0) {
+        b[0] = (m * stx - st * sx) / den;
+        b[1] = (m * sty - st * sy) / den;
+        b[2] = (m * stz - st * sz) / den;
+    }
+    const double a[3] = {(sx - b[0] * st) / m, (sy - b[1] * st) / m, (sz - b[2] * st) / m};
+    const float af[3] = {(float)a[0], (float)a[1], (float)a[2]}, bf[3] = {(float)b[0], (float)b[1], (float)b[2]};
+    double e64 = 0.0, e32 = 0.0;
+#pragma unroll
+    for (int i = 0; i <= TILE; i++) {
+        if (i < m) {
+            const double t = (k0 + i) - tc;
+            const double rx = px[i] - (a[0] + b[0] * t), ry = py[i] - (a[1] + b[1] * t), rz = pz[i] - (a[2] + b[2] * t);
+            e64 = fmax(e64, sqrt(rx * rx + ry * ry + rz * rz));
+            const double fx = px[i] - ((double)af[0] + (double)bf[0] * t), fy = py[i] - ((double)af[1] + (double)bf[1] * t),
+                         fz = pz[i] - ((double)af[2] + (double)bf[2] * t);
+            e32 = fmax(e32, sqrt(fx * fx + fy * fy + fz * fz));
+        }
+    }
+    // Loewner bound of the rotated covariances of the tile's points
+    double ref[6];
+    {
+        const double th = OMEGA_PT * kp;
+        rot_cov(trk + (size_t)kp * 12 + 6, cos(th), sin(th), ref);
+    }
+    double eta = 0.0, trmax = ref[0] + ref[3] + ref[5];
+#pragma unroll
+    for (int i = 0; i < TILE; i++) {
+        const int k = k0 + i;
+        if (k <= kp) {
+            const double th = OMEGA_PT * k;
+            double q[6];
+            rot_cov(trk + (size_t)k * 12 + 6, cos(th), sin(th), q);
+            double d[6];
+#pragma unroll
+            for (int u = 0; u < 6; u++) d[u] = q[u] - ref[u];
+            const double g0 = d[0] + fabs(d[1]) + fabs(d[2]), g1 = d[3] + fabs(d[1]) + fabs(d[4]),
+                         g2 = d[5] + fabs(d[2]) + fabs(d[4]);
+            eta = fmax(eta, fmax(g0, fmax(g1, g2)));
+            trmax = fmax(trmax, q[0] + q[3] + q[5]);
+        }
+    }
+    eta = eta * (1.0 + 1e-12) + 1e-13 * trmax;   // rounding of the rotations and differences
+    double U[6] = {ref[0] + eta + R2, ref[1], ref[2], ref[3] + eta + R2, ref[4], ref[5] + eta + R2};
+    r1.ax = af[0]; r1.ay = af[1]; r1.az = af[2];
+    r1.bx = bf[0]; r1.by = bf[1]; r1.bz = bf[2];
+    r1.e = up(e32 * (1.0 + 1e-6) + 1e-12);
+    r1.lam = up(lam_bound(U) * (1.0 + 1e-6));
+#pragma unroll
+    for (int u = 0; u < 3; u++) { r2.a[u] = a[u]; r2.b[u] = b[u]; }
+#pragma unroll
+    for (int u = 0; u < 6; u++) r2.U[u] = U[u];
+    r2.e = e64 * (1.0 + 1e-12) + 1e-15;
+    r2.R2 = R2;
+    r2.t1 = (double)(kl - k0) - 0.5 * TILE;
+    r2.pad = 0.0;
+}
+
+// The axis parameter range of tile j of a cluster with n points: t in [-4, t1].
+PHX_HDF float tile_t1(int n, int j) {
+    const int k0 = j * TILE, kl = k0 + TILE < n - 1 ? k0 + TILE : n - 1;
+    return (float)(kl - k0) - 0.5f * TILE;
+}
+
+// Square root to within 1 ulp (the hardware instruction on the device).
+#if defined(__HIP_DEVICE_COMPILE__)
+PHX_HDF float f_sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+#else
+PHX_HDF float f_sqrt_fast(float x) { return sqrtf(x); }
+#endif
+
+// Level 1, FP32.  kchi = sqrt(chi2_max (1 + CUT_REL)) rounded up.  The test accepts when
+// dist(0, axis segment) <= rho + kchi sqrt(lam_s + lam_c), compared as squares with relative margin 4e-5
+// and an absolute one of 1e-5 kpc (plus 2e-6 |da|^2 for the cancellation in the cross product); FP32
+// rounding of every step is far below both.
+PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi) {
+    const float dax = s.ax - c.ax, day = s.ay - c.ay, daz = s.az - c.az;
+    const float dbx = s.bx - c.bx, dby = s.by - c.by, dbz = s.bz - c.bz;
+    const float t0 = -0.5f * TILE;
+    const float dd = f_fma(dbx, dbx, f_fma(dby, dby, dbz * dbz));
+    const float ad = f_fma(dax, dbx, f_fma(day, dby, daz * dbz));
+    const float x0x = f_fma(dbx, t0, dax), x0y = f_fma(dby, t0, day), x0z = f_fma(dbz, t0, daz);
+    const float x1x = f_fma(dbx, t1, dax), x1y = f_fma(dby, t1, day), x1z = f_fma(dbz, t1, daz);
+    const float d0 = f_fma(x0x, x0x, f_fma(x0y, x0y, x0z * x0z));
+    const float d1 = f_fma(x1x, x1x, f_fma(x1y, x1y, x1z * x1z));
+    const float cx = f_fma(day, dbz, -daz * dby), cy = f_fma(daz, dbx, -dax * dbz), cz = f_fma(dax, dby, -day * dbx);
+    const float dl = f_fma(cx, cx, f_fma(cy, cy, cz * cz));   // |da x db|^2 = dd * (distance of the line)^2
+    const bool interior = (dd > 0.0f) & (-ad > t0 * dd) & (-ad < t1 * dd);
+    const float e = s.e + c.e;
+    const float na = fabsf(dax) + fabsf(day) + fabsf(daz), nb = fabsf(dbx) + fabsf(dby) + fabsf(dbz);
+    const float rho = e + (float)ROT_W * (na + 0.5f * TILE * nb + e);
+    const float lim = (rho + kchi * f_sqrt_fast(s.lam + c.lam)) * 1.00001f + 1e-5f;
+    const float lim2 = f_fma(lim, lim, 2e-6f * na * na);
+    const float lhs = interior ? dl : fminf(d0, d1);
+    const float rhs = interior ? lim2 * dd : lim2;
+    return lhs * 0.99996f <= rhs;
+}
+
+// Level 2, FP64.  chi2c = chi2_max (1 + CUT_REL).
+PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi2c) {
+    const double da0 = s.a[0] - c.a[0], da1 = s.a[1] - c.a[1], da2 = s.a[2] - c.a[2];
+    const double db0 = s.b[0] - c.b[0], db1 = s.b[1] - c.b[1], db2 = s.b[2] - c.b[2];
+    double U[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) U[u] = s.U[u] + c.U[u];
+    const double tr = U[0] + U[3] + U[5];
+    if (!(tr < 1e9 * (s.R2 + c.R2))) return true;   // condition number not bounded: never pruned here
+    const double dlt = 1e-13 * tr;                    // U rounded up over the rounding of the sum
+    U[0] += dlt; U[3] += dlt; U[5] += dlt;
+    const double na = sqrt(da0 * da0 + da1 * da1 + da2 * da2) * (1.0 + 1e-12);
+    const double nb = sqrt(db0 * db0 + db1 * db1 + db2 * db2) * (1.0 + 1e-12);
+    const double e = s.e + c.e;
+    const double rho = (e + ROT_W * (na + 0.5 * TILE * nb + e)) * (1.0 + 1e-12);
+    const double G0 = U[3] * U[5] - U[4] * U[4], G1 = U[2] * U[4] - U[1] * U[5], G2 = U[1] * U[4] - U[2] * U[3];
+    const double G3 = U[0] * U[5] - U[2] * U[2], G4 = U[1] * U[2] - U[0] * U[4], G5 = U[0] * U[3] - U[1] * U[1];
+    const double det = U[0] * G0 + U[1] * G1 + U[2] * G2;
+    if (!(det > 0.0)) return true;
+    const double ga0 = G0 * da0 + G1 * da1 + G2 * da2, ga1 = G1 * da0 + G3 * da1 + G4 * da2,
+                 ga2 = G2 * da0 + G4 * da1 + G5 * da2;
+    const double gb0 = G0 * db0 + G1 * db1 + G2 * db2, gb1 = G1 * db0 + G3 * db1 + G4 * db2,
+                 gb2 = G2 * db0 + G4 * db1 + G5 * db2;
+    const double B = da0 * gb0 + da1 * gb1 + da2 * gb2, C = db0 * gb0 + db1 * gb1 + db2 * gb2;
+    const double t0 = -0.5 * TILE;
+    double th = C > 0.0 ? -B / C : t0;
+    th = fmin(fmax(th, t0), t1);
+    const double z0 = ga0 + th * gb0, z1 = ga1 + th * gb1, z2 = ga2 + th * gb2;
+    const double za = z0 * da0 + z1 * da1 + z2 * da2, zb = z0 * db0 + z1 * db1 + z2 * db2;
+    const double lin = fmin(za + t0 * zb, za + t1 * zb);
+    const double nz2 = z0 * z0 + z1 * z1 + z2 * z2;
+    const double nz = sqrt(nz2) * (1.0 + 1e-12);
+    const double u0 = U[0] * z0 + U[1] * z1 + U[2] * z2, u1 = U[1] * z0 + U[3] * z1 + U[4] * z2,
+                 u2 = U[2] * z0 + U[4] * z1 + U[5] * z2;
+    const double zuz = z0 * u0 + z1 * u1 + z2 * u2;
+    const double thr = chi2c * det * det;
+    const double lbd = 2.0 * det * (lin - rho * nz) - zuz;   // det^2 x (the lower bound)
+    const double err = 1e-12 * (2.0 * det * (nz * (na + 0.5 * TILE * nb) + rho * nz) + tr * nz2 + thr);
+    return !(lbd - thr > err);
+}
+
+}  // namespace phx

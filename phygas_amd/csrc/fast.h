@@ -10,7 +10,6 @@ struct FastTables {
     void *mem[16] = {};
     size_t bytes[16] = {};
     int nc = 0, npts = 0, ntiles = 0, nc_pad = 0;
-    double chi2_built = -1.0;          // chi2_max the cluster tiles were built for
     const int *d_cn = nullptr;         // resident cluster point counts and kernel radii (owned by the context)
     const double *d_R2 = nullptr;
 };
@@ -19,8 +18,8 @@ struct FastTables {
 struct FastWork {
     void *mem[16] = {};
     size_t bytes[16] = {};
-    long cand_cap = 0, kept_cap = 0, row_cap = 0;
-    long last_cand = 0;
+    long cand_cap = 0, kept_cap = 0, row_cap = 0, item_cap = 0;
+    long last_cand = 0, last_items = 0;
     int scan_blocks = 0;               // grid of the persistent tile scan (0: default)
     // hipGraph of the pass's launches, replayed while sizes, parameters and buffers stay the same
     bool use_graphs = false;

@@ -6,58 +6,60 @@
 //   2. k_lin_fast    tracks: FP64 nominal orbit, the six neighbour offsets of the finite-difference STM
 //                    propagated with the FP32 difference force (force32.h) and FP64 accumulators,
 //                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.
-//   3. k_star_tiles  per (star, 8-point tile): bounding box of the positions in a frame co-rotating with
-//                    the Galaxy, and the radius term of the cut (below)
+//   3. k_star_recs   per (star, 8-point tile): the tile records of cull.h (axis line, tube radius and
+//                    Loewner covariance bound in the co-rotating frame)
 //   4. k_scan_tiles  persistent, length-bucketed: each wavefront takes 64 clusters (sorted by track length)
-//                    x 32 stars, tests every (pair, tile) and appends the pairs with any tile that may hold
-//                    a point under the cut, with a bit mask of those tiles
-//   5. k_refine      one lane per candidate pair: FP32 point test, then the reference's FP64 evaluation
-//                    (two Mahalanobis forms per point, strict-< first-index minimum) on surviving points;
-//                    pairs with chi2 < chi2_max are appended to the kept list
+//                    x 32 stars, tests every (pair, tile) with the FP32 tube test and the survivors with the
+//                    FP64 Mahalanobis certificate (cull.h), and appends the pairs with any tile left, with a
+//                    bit mask of those tiles
+//   5. k_expand      one item per (candidate, tile left), contiguous per candidate
+//      k_items       one lane per item: the reference's FP64 evaluation of the tile's points (two
+//                    Mahalanobis forms per point, strict-< first-index minimum)
+//      k_reduce      one lane per candidate: minimum over its items; pairs with chi2 < chi2_max are appended
+//                    to the kept list
 //   6. k_finish_fast one lane per kept pair: the 512 Box-Muller draws of the pair's seed, screened draw
 //                    by draw (below), exact counts; rows with p_r50 >= p_min appended
 //
-// Exactness of the cut.  For a point k of a pair, chi2_k = min(q(m0), q(ms)) >= d^2 / lambda_max(M_k) with
-// d the distance of the step's segment m0 -> m1 from the origin (reference proof: src/phygas.c:360-369), and
-// lambda_max(M_k) <= lambda_s + lambda_c + R^2 (Weyl) with lambda_o an upper bound of the largest eigenvalue
-// of each covariance (min of the trace and the Gershgorin bound).  A tile's boxes are built from the points
-// rotated by R_k (rotation angle w*k about the Galactic z axis); rotating both ends of a segment by R_k moves
-// the far end by at most w*|m1| from the box, so d >= gap - w (gap + diag_s + diag_c), gap = distance of the
-// origin from box_s - box_c.  When that exceeds sqrt(chi2_max) (sigma_s + sigma_c + R) every point of the
-// tile has chi2 > chi2_max and cannot be the minimum of a kept pair.  Float rounding is covered by outward
-// rounding of the boxes and a 1e-5 relative + 1e-5 kpc margin.  Kept pairs get the FP64 chi2, kb and sb of
-// a full scan of the same tracks.
+// Exactness of the cut: a tile dropped by the scan holds no point with chi2 < chi2_max (cull.h), and the
+// refine's point test drops a point only when chi2 >= d^2 / lambda_max(M_k) > chi2_max, with d the distance
+// of the step's segment from the origin (reference proof: src/phygas.c:360-369) and lambda_max(M_k) <=
+// lambda_s + lambda_c + R^2 (Weyl).  Kept pairs get the FP64 chi2, kb and sb of a full scan of the same
+// tracks.
 //
 // Exactness of the finish.  splitmix64's state after i calls is seed + i*gamma, so draw i's six uniforms
 // are known by index.  x0 = m0 + l00 z0 needs |x0| < r to count, i.e. z0 in an interval I; z0 = rho cos(.)
 // with rho = sqrt(-2 ln u1), so a draw whose u1 exceeds exp(-dist(0, I)^2 / 2) cannot count and its other
-// five uniforms are never generated.  Otherwise z0 is estimated in FP32 (hardware log2 / cos) and only
-// draws whose estimate falls within 1e-3 of I are evaluated exactly (the reference's FP64 arithmetic).
+// five uniforms are never generated.  Otherwise z0 is estimated in FP32 (hardware log2 / cos); a draw whose
+// estimate is more than 1e-3 outside I cannot count.  For the rest all three z are estimated the same way,
+// and the draw is counted (or not) from the estimated position when it lies more than 1e-3 sum|L| from the
+// counting radius; only draws within that band are evaluated exactly (the reference's FP64 arithmetic).
+// The 1e-3 bound on |z estimate - z| holds with a factor ~40 to spare (phx_check_finish_screen measures it,
+// tail draws included).
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "cull.h"
 #include "fast.h"
 #include "force32.h"
 
 namespace phx {
 
 // ------------------------------------------------------------------------------------------- layout
-constexpr int TILE = 8;                 // points per culling tile
 constexpr int SPW = 32;                 // stars per wavefront in the tile scan
-constexpr double OMEGA_PT = -0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122;   // frame rotation per 0.5-Myr point
-constexpr float ROT_SLACK = (float)(0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122 * 1.000001);
-constexpr float REL_MARGIN = 1e-5f, ABS_MARGIN = 1e-5f;
 
 struct Cand { int star, cluster; unsigned long long mask; };
 struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
-enum { W_PS0, W_SIG, W_TRK, W_TRKF, W_STILE, W_CAND, W_KEPT, W_ROWS, W_CNT, W_NSLOT };
+enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
+       W_TILED, W_NSLOT };
+// Counters (W_CNT): candidates, kept pairs, rows, items, scan task
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_N };
 // Table buffers (FastTables::mem slots)
-enum { T_CTRK, T_CTRKF, T_CTLO, T_CTHI, T_CNT, T_CPERM, T_NSLOT };
+enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_NSLOT };
 
 static int ensure(void *&p, size_t &have, size_t need) {
     if (need <= have && p) return 0;
@@ -67,29 +69,6 @@ static int ensure(void *&p, size_t &have, size_t need) {
     PHX_CK(hipMalloc(&p, need ? need : 8));
     have = need ? need : 8;
     return 0;
-}
-
-// Largest-eigenvalue upper bound of a symmetric PSD 3x3 (xx xy xz yy yz zz): min(trace, Gershgorin).
-__device__ __forceinline__ double lam_bound(const double *S) {
-    double tr = S[0] + S[3] + S[5];
-    double g0 = S[0] + fabs(S[1]) + fabs(S[2]), g1 = S[3] + fabs(S[1]) + fabs(S[4]), g2 = S[5] + fabs(S[2]) + fabs(S[4]);
-    double g = fmax(g0, fmax(g1, g2));
-    return fmax(fmin(tr, g), 0.0);
-}
-// Neighbouring floats (finite inputs): one step towards +inf / -inf.
-__device__ __forceinline__ float next_up(float f) {
-    if (f == 0.0f) return __int_as_float(1);
-    const int b = __float_as_int(f);
-    return __int_as_float(f > 0.0f ? b + 1 : b - 1);
-}
-__device__ __forceinline__ float next_down(float f) { return -next_up(-f); }
-__device__ __forceinline__ float up(double v) {   // float >= v
-    float f = (float)v;
-    return (double)f < v ? next_up(f) : f;
-}
-__device__ __forceinline__ float down(double v) {   // float <= v
-    float f = (float)v;
-    return (double)f > v ? next_down(f) : f;
 }
 
 // ------------------------------------------------------------------------------------------- FP64 nominal
@@ -114,11 +93,9 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, V3 p) {
 
 // ------------------------------------------------------------------------------------------- tracks
 // Two lanes per star; lane q propagates offsets 3q .. 3q+2 (x, y, z for q = 0; vx, vy, vz for q = 1) and
-// both lanes run the same FP64 nominal.  Track layout as the plain path: [star][npts][12] FP64, plus
-// [star][npts] float4 (x, y, z, lambda bound of S) for the cut.
+// both lanes run the same FP64 nominal.  Track layout as the plain path: [star][npts][12] FP64.
 __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
-                                                  double *__restrict__ trk, float4 *__restrict__ trkf, Pot P, PotF Q,
-                                                  int npts, int n) {
+                                                  double *__restrict__ trk, Pot P, PotF Q, int npts, int n) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = gid >> 1, q = gid & 1;
     const bool live = s < n;
@@ -203,53 +180,27 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
         out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = v.x; out[4] = v.y; out[5] = v.z;
 #pragma unroll
         for (int i = 0; i < 6; i++) out[6 + i] = S[i];
-        trkf[(size_t)s * npts + k] = make_float4((float)x.x, (float)x.y, (float)x.z, up(lam_bound(S) * (1.0 + 1e-6)));
     }
 }
 
-// ------------------------------------------------------------------------------------------- tiles
-// Tile j of an object covers points [jT, jT+T) of its scan and its box the points [jT, min(jT+T, n-1)]
-// (each step's far end).  lo.w = e = (sqrt(chi2_max) sigma + w diag) / (1 - w), hi.w = diag.
-__device__ void tile_box(const double *pos, size_t stride, int n, int j, double lam_sqrt_max, float kchi,
-                         float4 &lo, float4 &hi) {
-    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    const int k0 = j * TILE, k1 = min(k0 + TILE, n - 1);
-    for (int k = k0; k <= k1; k++) {
-        const double *p = pos + (size_t)k * stride;
-        const double ph = OMEGA_PT * k, c = cos(ph), sn = sin(ph);
-        const double rx = c * p[0] - sn * p[1], ry = sn * p[0] + c * p[1], rz = p[2];
-        mn[0] = fmin(mn[0], rx); mn[1] = fmin(mn[1], ry); mn[2] = fmin(mn[2], rz);
-        mx[0] = fmax(mx[0], rx); mx[1] = fmax(mx[1], ry); mx[2] = fmax(mx[2], rz);
-    }
-    const double dg = sqrt((mx[0] - mn[0]) * (mx[0] - mn[0]) + (mx[1] - mn[1]) * (mx[1] - mn[1]) +
-                           (mx[2] - mn[2]) * (mx[2] - mn[2]));
-    const double w = ROT_SLACK;
-    const double e = ((double)kchi * lam_sqrt_max + w * dg) / (1.0 - w);
-    lo = make_float4(down(mn[0]), down(mn[1]), down(mn[2]), up(e * (1.0 + REL_MARGIN) + ABS_MARGIN));
-    hi = make_float4(up(mx[0]), up(mx[1]), up(mx[2]), up(dg));
-}
-
-// One lane per (star, tile).
-__global__ __launch_bounds__(256) void k_star_tiles(const double *__restrict__ trk, const float4 *__restrict__ trkf,
-                                                    int npts, int ntiles, int n, float kchi, float4 *__restrict__ tlo,
-                                                    float4 *__restrict__ thi) {
+// ------------------------------------------------------------------------------------------- tile records
+// One lane per (star, tile): records of the star tracks [ns][npts][12] into srec1 / srec2 [ns_pad][ntl]; rows
+// past the last star hold records no pair test accepts (the scan reads whole blocks of SPW stars).
+__global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ trk, int npts, int ntl, int n, int n_pad,
+                                                   TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)n * ntiles) return;
-    const int s = (int)(id / ntiles), j = (int)(id % ntiles);
-    const int k0 = j * TILE, k1 = min(k0 + TILE, npts) - 1;
-    double lam = 0.0;
-    for (int k = k0; k <= k1; k++) lam = fmax(lam, (double)trkf[(size_t)s * npts + k].w);
-    float4 lo, hi;
-    tile_box(trk + (size_t)s * npts * 12, 12, npts, j, sqrt(lam), kchi, lo, hi);
-    tlo[id] = lo;
-    thi[id] = hi;
+    if (id >= (long)n_pad * ntl) return;
+    const int s = (int)(id / ntl), j = (int)(id % ntl);
+    TileRec1 a = {1e30f, 1e30f, 1e30f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    TileRec2 b = {};
+    if (s < n) tile_records(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b);
+    r1[id] = a;
+    r2[id] = b;
 }
 
-// Cluster tables for the fast form: AoS FP64 tracks [c][npts][12], float4 tracks [c][npts], and tiles in
-// structure-of-arrays [tile][nc_pad] over clusters sorted by track length (longest first).
+// Cluster tables for the fast form: AoS FP64 tracks [c][npts][12].
 __global__ __launch_bounds__(256) void k_cluster_aos(const double *__restrict__ csoa, const int *__restrict__ cn,
-                                                     int nc, int npts, double *__restrict__ ctrk,
-                                                     float4 *__restrict__ ctrkf) {
+                                                     int nc, int npts, double *__restrict__ ctrk) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)nc * npts) return;
     const int c = (int)(id / npts), k = (int)(id % npts);
@@ -259,78 +210,90 @@ __global__ __launch_bounds__(256) void k_cluster_aos(const double *__restrict__ 
     double *o = ctrk + id * 12;
 #pragma unroll
     for (int q = 0; q < 12; q++) o[q] = v[q];
-    ctrkf[id] = make_float4((float)v[0], (float)v[1], (float)v[2], up(lam_bound(v + 6) * (1.0 + 1e-6)));
 }
 
-__global__ __launch_bounds__(256) void k_cluster_tiles(const double *__restrict__ ctrk,
-                                                       const float4 *__restrict__ ctrkf, const int *__restrict__ cn,
-                                                       const double *__restrict__ R2, const int *__restrict__ perm,
-                                                       int nc, int nc_pad, int npts, int ntiles, float kchi,
-                                                       float4 *__restrict__ tlo, float4 *__restrict__ thi) {
+// Cluster tile records in structure-of-arrays [tile][nc_pad] over clusters sorted by track length (longest
+// first); the pair's kernel radius enters through the cluster (R^2 I added to U).
+__global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__ ctrk, const int *__restrict__ cn,
+                                                      const double *__restrict__ R2, const int *__restrict__ perm,
+                                                      int nc, int nc_pad, int npts, int ntl,
+                                                      TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)nc_pad * ntiles) return;
+    if (id >= (long)nc_pad * ntl) return;
     const int j = (int)(id / nc_pad), cs = (int)(id % nc_pad);
-    float4 lo = make_float4(1e30f, 1e30f, 1e30f, 0.f), hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+    TileRec1 a = {};
+    TileRec2 b = {};
     if (cs < nc) {
         const int c = perm[cs], n = cn[c];
-        if (j * TILE < n) {
-            const int k0 = j * TILE, k1 = min(k0 + TILE, n) - 1;
-            double lam = 0.0;
-            for (int k = k0; k <= k1; k++) lam = fmax(lam, (double)ctrkf[(size_t)c * npts + k].w);
-            tile_box(ctrk + (size_t)c * npts * 12, 12, n, j, sqrt(lam), kchi, lo, hi);
-            // the kernel radius R of the pair enters through the cluster: e += sqrt(chi2_max) R / (1 - w)
-            const double w = ROT_SLACK;
-            lo.w = up(((double)lo.w + (double)kchi * sqrt(R2[c]) / (1.0 - w)) * (1.0 + REL_MARGIN));
-        }
+        if (j * TILE < n) tile_records(ctrk + (size_t)c * npts * 12, n, j, R2[c], a, b);
     }
-    tlo[(size_t)j * nc_pad + cs] = lo;
-    thi[(size_t)j * nc_pad + cs] = hi;
+    r1[id] = a;
+    r2[id] = b;
 }
 
 // ------------------------------------------------------------------------------------------- tile scan
-// Persistent: each wavefront loops over tasks (cluster block of 64 sorted clusters, star block of SPW
-// stars); tasks are ordered longest clusters first.
-__global__ __launch_bounds__(256) void k_scan_tiles(const float4 *__restrict__ ctlo, const float4 *__restrict__ cthi,
-                                                    const int *__restrict__ cnt_sorted, const int *__restrict__ perm,
-                                                    int nc, int nc_pad, const float4 *__restrict__ stlo,
-                                                    const float4 *__restrict__ sthi, int ns, int ntiles_s,
-                                                    Cand *__restrict__ cand, unsigned long long *__restrict__ ncand,
-                                                    long cap) {
+// Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) from
+// a counter until none is left, lane = cluster.  Per tile j: level 1 for every star of the block (the star's record is wave-uniform and
+// read through scalar loads), then level 2 for the lane's level-1 survivors.  Pairs with any tile left are
+// appended with the bit mask of those tiles.
+__global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__ crec1,
+                                                    const TileRec2 *__restrict__ crec2,
+                                                    const int *__restrict__ ntc_sorted,
+                                                    const int *__restrict__ cn_sorted, const int *__restrict__ perm,
+                                                    int nc, int nc_pad, const TileRec1 *__restrict__ srec1,
+                                                    const TileRec2 *__restrict__ srec2, int ns, int ntl,
+                                                    float kchi, double chi2c, Cand *__restrict__ cand,
+                                                    unsigned long long *__restrict__ ncand, long cap,
+                                                    unsigned long long *__restrict__ next_task) {
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
-    const long ntasks = (long)ncb * nsb;
-    for (long task = wave; task < ntasks; task += nwaves) {
-        const int cb = (int)(task / nsb), sb = (int)(task % nsb);
+    const int ntasks = ncb * nsb;
+    for (;;) {
+        // tasks in star-block-major order, taken dynamically: the waves in flight share few star blocks
+        int t = 0;
+        if (lane == 0) t = (int)atomicAdd(next_task, 1ull);
+        const int task = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        if (task >= ntasks) break;
+        const int sb = task / ncb, cb = task % ncb;
         const int cs = cb * 64 + lane;
-        const int nt = cs < nc ? cnt_sorted[cs] : 0;
-        const int ntmax = __shfl(nt, 0, 64);   // sorted descending: lane 0 has the block's longest
+        const int nt = cs < nc ? ntc_sorted[cs] : 0;
+        const int ncl = cs < nc ? cn_sorted[cs] : 0;
+        const int ntmax = __builtin_amdgcn_readfirstlane(nt);   // sorted descending: lane 0 has the longest
         const int s0 = sb * SPW;
-        const int nsl = min(SPW, ns - s0);
         unsigned long long mask[SPW];
 #pragma unroll
         for (int i = 0; i < SPW; i++) mask[i] = 0ull;
         for (int j = 0; j < ntmax; j++) {
-            const float4 cl = ctlo[(size_t)j * nc_pad + cs], ch = cthi[(size_t)j * nc_pad + cs];
             const bool cv = j < nt;
+            const TileRec1 c1 = crec1[(size_t)j * nc_pad + cs];
+            const float t1 = cv ? tile_t1(ncl, j) : 0.0f;
+            unsigned l1 = 0u;
 #pragma unroll
-            for (int i = 0; i < SPW; i++) {
-                if (i < nsl) {
-                    const size_t si = (size_t)(s0 + i) * ntiles_s + j;
-                    const float4 sl = stlo[si], sh = sthi[si];
-                    const float gx = fmaxf(fmaxf(sl.x - ch.x, cl.x - sh.x), 0.f);
-                    const float gy = fmaxf(fmaxf(sl.y - ch.y, cl.y - sh.y), 0.f);
-                    const float gz = fmaxf(fmaxf(sl.z - ch.z, cl.z - sh.z), 0.f);
-                    const float g2 = f_fma(gx, gx, f_fma(gy, gy, gz * gz));
-                    const float t = sl.w + cl.w;
-                    if (cv && g2 <= t * t * (1.0f + REL_MARGIN)) mask[i] |= 1ull << j;
+            for (int i0 = 0; i0 < SPW; i0 += 8) {   // star records of the padded block: wave-uniform loads
+                TileRec1 sr[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) sr[u] = srec1[(size_t)(s0 + i0 + u) * ntl + j];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (l1_pass(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
+            }
+            if (!cv) l1 = 0u;
+            if (l1) {
+                const TileRec2 c2 = crec2[(size_t)j * nc_pad + cs];
+                unsigned l2 = 0u;
+                while (l1) {
+                    const int i = __builtin_ctz(l1);
+                    l1 &= l1 - 1u;
+                    const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
+                    if (l2_pass(s2, c2, (double)t1, chi2c)) l2 |= 1u << i;
                 }
+#pragma unroll
+                for (int i = 0; i < SPW; i++) mask[i] |= (unsigned long long)((l2 >> i) & 1u) << j;
             }
         }
 #pragma unroll
         for (int i = 0; i < SPW; i++) {
-            if (i < nsl) {   // wave-uniform
+            if (s0 + i < ns) {   // wave-uniform
                 const bool hit = cs < nc && mask[i] != 0ull;
                 const unsigned long long b = __ballot(hit);
                 if (b != 0ull) {
@@ -348,70 +311,200 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const float4 *__restrict__ c
 }
 
 // ------------------------------------------------------------------------------------------- refine
-__global__ __launch_bounds__(256) void k_refine(const Cand *__restrict__ cand,
-                                                const unsigned long long *__restrict__ ncand, long cand_cap,
-                                                const double *__restrict__ strk, const float4 *__restrict__ strkf,
-                                                const double *__restrict__ ctrk, const float4 *__restrict__ ctrkf,
-                                                const int *__restrict__ cn, const double *__restrict__ R2s, int npts,
-                                                double chi2_max, Kept *__restrict__ kept,
-                                                unsigned long long *__restrict__ nkept, long kept_cap) {
-    const long nc_ = min((long)*ncand, cand_cap);
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < nc_; id += (long)gridDim.x * blockDim.x) {
-        const Cand cd = cand[id];
-        const int s = cd.star, c = cd.cluster;
-        const int n = min(npts, cn[c]);
-        const double R2 = R2s[c];
-        const float R2f = (float)(R2 * (1.0 + 1e-6)) + 1e-30f;
-        const float4 *sf = strkf + (size_t)s * npts;
-        const float4 *cf = ctrkf + (size_t)c * npts;
-        const double *sd = strk + (size_t)s * npts * 12;
-        const double *cdp = ctrk + (size_t)c * npts * 12;
-        double best = __builtin_inf(), sbest = 0.0;
-        int kb = 0;
-        unsigned long long mask = cd.mask;
-        while (mask) {
-            const int j = __builtin_ctzll(mask);
-            mask &= mask - 1ull;
-            const int k0 = j * TILE, k1 = min(k0 + TILE, n);
-            for (int k = k0; k < k1; k++) {
-                // FP32 point test: distance of the step's segment from the origin vs sqrt(chi2_max lambda)
-                const float4 a0 = sf[k], b0 = cf[k];
-                const float m0x = a0.x - b0.x, m0y = a0.y - b0.y, m0z = a0.z - b0.z;
-                float d2 = f_fma(m0x, m0x, f_fma(m0y, m0y, m0z * m0z));
-                if (k + 1 < n) {
-                    const float4 a1 = sf[k + 1], b1 = cf[k + 1];
-                    const float rx = (a1.x - b1.x) - m0x, ry = (a1.y - b1.y) - m0y, rz = (a1.z - b1.z) - m0z;
-                    const float dd = f_fma(rx, rx, f_fma(ry, ry, rz * rz));
-                    const float ad = f_fma(m0x, rx, f_fma(m0y, ry, m0z * rz));
-                    if (ad < 0.f && dd > 0.f) {
-                        const float sp = fminf(-ad / dd, 1.f);
-                        const float px = f_fma(rx, sp, m0x), py = f_fma(ry, sp, m0y), pz = f_fma(rz, sp, m0z);
-                        d2 = f_fma(px, px, f_fma(py, py, pz * pz));
-                    }
-                }
-                const float lim = f_sqrt((float)chi2_max * (a0.w + b0.w + R2f)) * (1.0f + REL_MARGIN) + ABS_MARGIN;
-                if (f_sqrt(d2) > lim) continue;
-                // the reference's FP64 evaluation of the point (reference: src/phygas.c:395-403)
-                const double *ps = sd + (size_t)k * 12, *pc = cdp + (size_t)k * 12;
-                double M[6];
+// The reference's FP64 value of point k of a pair (reference: src/phygas.c:395-403): min of the Mahalanobis
+// form at the step's start m0 and at the point of the step's segment m0 -> m1 closest to the origin (that
+// one only when >= 0 and smaller), -1 when neither form is defined.  *sv: the segment parameter of the
+// chosen form (0 for m0).  Ms: S_star + S_cluster (xx xy xz yy yz zz) before the kernel term R^2.
+__device__ __forceinline__ double point_chi2(V3 m0, V3 m1, bool has_next, const double *Ms, double R2, double *sv_out) {
+    double M[6];
 #pragma unroll
-                for (int i = 0; i < 6; i++) M[i] = ps[6 + i] + pc[6 + i];
-                M[0] += R2; M[3] += R2; M[5] += R2;
-                const V3 m0 = v3(ps[0] - pc[0], ps[1] - pc[1], ps[2] - pc[2]);
-                double qv = mahalanobis(M, m0.x, m0.y, m0.z), sv = 0.0;
-                if (k + 1 < n) {
-                    const V3 m1 = v3(ps[12] - pc[12], ps[13] - pc[13], ps[14] - pc[14]);
-                    sv = seg_param(m0, m1);
-                    const V3 ms = m0 + (m1 - m0) * sv;
-                    const double qs = mahalanobis(M, ms.x, ms.y, ms.z);
-                    if (qs >= 0 && qs < qv) qv = qs; else sv = 0.0;
-                }
-                if (qv >= 0 && qv < best) { best = qv; kb = k; sbest = sv; }
+    for (int i = 0; i < 6; i++) M[i] = Ms[i];
+    M[0] += R2; M[3] += R2; M[5] += R2;
+    double qv = mahalanobis(M, m0.x, m0.y, m0.z), sv = 0.0;
+    if (has_next) {
+        sv = seg_param(m0, m1);
+        const V3 ms = m0 + (m1 - m0) * sv;
+        const double qs = mahalanobis(M, ms.x, ms.y, ms.z);
+        if (qs >= 0 && qs < qv) qv = qs; else sv = 0.0;
+    }
+    *sv_out = sv;
+    return qv;
+}
+
+// Tiled tracks for the pair evaluation: per object and 8-point tile j, 75 doubles in structure-of-arrays
+// order: x, y, z of the positions 8j .. 8j+8 (9 each: the last is the far end of point 8j+7's segment),
+// then the six covariance entries of the points 8j .. 8j+7 (8 each).  Eight lanes reading the same field of
+// eight consecutive points read 64 contiguous bytes.  Positions past the track repeat its last position and
+// covariances past it are 0 (never read for a pair: its points stop at the cluster's length).
+constexpr int TREC = 75;
+__global__ __launch_bounds__(256) void k_tiled(const double *__restrict__ trk, const int *__restrict__ cn, int n_const,
+                                               int nobj, int npts, int ntl, double *__restrict__ out) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nobj * ntl * TREC) return;
+    const int i = (int)(id % TREC);
+    const long ot = id / TREC;
+    const int j = (int)(ot % ntl), o = (int)(ot / ntl);
+    const int n = cn ? cn[o] : n_const;
+    const double *t = trk + (size_t)o * npts * 12;
+    double v;
+    if (i < 27) {
+        const int f = i / 9, k = j * TILE + i % 9;
+        v = t[(size_t)min(k, n - 1) * 12 + f];
+    } else {
+        const int f = (i - 27) / TILE, k = j * TILE + (i - 27) % TILE;
+        v = k < n ? t[(size_t)k * 12 + 6 + f] : 0.0;
+    }
+    out[id] = v;
+}
+
+// Point 8j + p of a pair from the tiled tracks of the star (ts) and the cluster (tc) at tile j: m0, m1 and
+// S_s + S_c.
+__device__ __forceinline__ void tiled_point(const double *__restrict__ ts, const double *__restrict__ tc, int p, V3 &m0,
+                                            V3 &m1, double *M) {
+    m0 = v3(ts[p] - tc[p], ts[9 + p] - tc[9 + p], ts[18 + p] - tc[18 + p]);
+    m1 = v3(ts[p + 1] - tc[p + 1], ts[10 + p] - tc[10 + p], ts[19 + p] - tc[19 + p]);
+#pragma unroll
+    for (int f = 0; f < 6; f++) M[f] = ts[27 + f * TILE + p] + tc[27 + f * TILE + p];
+}
+
+struct Item { int cand, star, cluster, j; };
+struct IRes { double q; int k, pad; };
+
+// Items of the candidates, one per tile left by the scan, contiguous per candidate in increasing tile
+// order: istart[c] = first item of candidate c.  Grid-stride over blocks of 256 candidates; each block takes
+// its range of the item list with one atomic.
+__global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
+                                                long cand_cap, Item *__restrict__ items,
+                                                unsigned long long *__restrict__ nitems, long item_cap,
+                                                long *__restrict__ istart) {
+    __shared__ unsigned wsum[4];
+    __shared__ unsigned long long base_s;
+    const long n = min((long)*ncand, cand_cap);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += (long)gridDim.x * blockDim.x) {
+        const long id = b0 + threadIdx.x;
+        const Cand cd = id < n ? cand[id] : Cand{0, 0, 0ull};
+        const unsigned cnt = (unsigned)__popcll(cd.mask);
+        unsigned x = cnt;   // inclusive scan within the wavefront
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        unsigned off = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            off += i < w ? wsum[i] : 0u;
+            tot += wsum[i];
+        }
+        if (threadIdx.x == 0) base_s = atomicAdd(nitems, (unsigned long long)tot);
+        __syncthreads();
+        const unsigned long long first = base_s + off + x - cnt;
+        if (id < n) {
+            istart[id] = (long)first;
+            unsigned long long mm = cd.mask, o = first;
+            while (mm) {
+                const int j = __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                if ((long)o < item_cap) items[o] = Item{(int)id, cd.star, cd.cluster, j};
+                o++;
             }
         }
-        if (best < chi2_max) {
-            const unsigned long long o = atomicAdd(nkept, 1ull);
-            if ((long)o < kept_cap) kept[o] = Kept{s, c, kb, 0, sbest, best};
+        __syncthreads();   // wsum / base_s reused by the next block of candidates
+    }
+}
+
+// Eight lanes per item, one per point of the tile: the reference's FP64 evaluation, then the strict-<
+// first-index minimum over the tile (lexicographic (chi2, k) across the eight lanes).  The item list is cut
+// into 8 contiguous chunks, chunk x taken by the workgroups the dispatcher places on XCD x (workgroup b on
+// XCD b mod 8), so that each XCD works through the items of a range of scan tasks.
+__global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, const unsigned long long *__restrict__ nitems,
+                                               long item_cap, const double *__restrict__ stile,
+                                               const double *__restrict__ ctile, const int *__restrict__ cn,
+                                               const double *__restrict__ R2s, int npts, int ntl,
+                                               IRes *__restrict__ res) {
+    const long n = min((long)*nitems, item_cap);
+    const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+    const long c0 = n * x / 8, c1 = n * (x + 1) / 8;
+    const int p = threadIdx.x & 7;
+    const int gpb = blockDim.x >> 3;   // items per workgroup pass
+    for (long b0 = c0 + (long)(blockIdx.x >> 3) * gpb; b0 < c1; b0 += (long)nbx * gpb) {
+        const long id = b0 + (threadIdx.x >> 3);
+        double q = __builtin_inf();
+        int k = 0;
+        if (id < c1) {
+            const Item it = items[id];
+            const int np = min(npts, cn[it.cluster]);
+            k = it.j * TILE + p;
+            if (k < np) {
+                V3 m0, m1;
+                double M[6], sv;
+                tiled_point(stile + ((size_t)it.star * ntl + it.j) * TREC, ctile + ((size_t)it.cluster * ntl + it.j) * TREC,
+                            p, m0, m1, M);
+                const double v = point_chi2(m0, m1, k + 1 < np, M, R2s[it.cluster], &sv);
+                if (v >= 0) q = v;
+            }
+        }
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+            const double qo = __shfl_xor(q, d, 64);
+            const int ko = __shfl_xor(k, d, 64);
+            if (qo < q || (qo == q && ko < k)) { q = qo; k = ko; }
+        }
+        if (p == 0 && id < c1) res[id] = IRes{q, k, 0};
+    }
+}
+
+// One lane per candidate: minimum over its items in tile order; kept pairs get the segment parameter of
+// their minimum re-evaluated and are appended to the kept list (one atomic per wavefront).
+__global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
+                                                long cand_cap, const long *__restrict__ istart,
+                                                const IRes *__restrict__ res, const unsigned long long *__restrict__ nitems,
+                                                long item_cap, const double *__restrict__ stile,
+                                                const double *__restrict__ ctile, const int *__restrict__ cn,
+                                                const double *__restrict__ R2s, int npts, int ntl, double chi2_max,
+                                                Kept *__restrict__ kept, unsigned long long *__restrict__ nkept,
+                                                long kept_cap) {
+    const long n = min((long)*ncand, cand_cap);
+    if ((long)*nitems > item_cap) return;   // the pass is run again with a larger item list
+    const int lane = threadIdx.x & 63;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long b0 = (long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {   // wave-uniform
+        const long id = b0 + lane;
+        Kept kp = {};
+        bool hit = false;
+        if (id < n) {
+            const Cand cd = cand[id];
+            const long i0 = istart[id];
+            const int ni = __popcll(cd.mask);
+            double best = __builtin_inf();
+            int kb = 0;
+            for (int i = 0; i < ni; i++) {
+                const IRes r = res[i0 + i];
+                if (r.q < best) { best = r.q; kb = r.k; }
+            }
+            if (best < chi2_max) {
+                const int c = cd.cluster, np = min(npts, cn[c]), j = kb / TILE;
+                V3 m0, m1;
+                double M[6], sv;
+                tiled_point(stile + ((size_t)cd.star * ntl + j) * TREC, ctile + ((size_t)c * ntl + j) * TREC, kb % TILE, m0,
+                            m1, M);
+                (void)point_chi2(m0, m1, kb + 1 < np, M, R2s[c], &sv);
+                kp = Kept{cd.star, c, kb, 0, sv, best};
+                hit = true;
+            }
+        }
+        const unsigned long long b = __ballot(hit);
+        if (b != 0ull) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(nkept, (unsigned long long)__popcll(b));
+            base = __shfl(base, 0, 64);
+            if (hit) {
+                const unsigned long long o = base + __popcll(b & ((1ull << lane) - 1ull));
+                if ((long)o < kept_cap) kept[o] = kp;
+            }
         }
     }
 }
@@ -425,8 +518,10 @@ __device__ __forceinline__ float z0_estimate(double u1, double u2) {
     return rho * c;
 }
 
-// Count draws with |m + L z| < r among the pair's 512 (exact), screening with the x0 interval.
-__device__ int count_inside(uint64_t seed, const double l[6], V3 m, double r, double r_alt, int *n_alt) {
+// Count draws with |m + L z| < r among the pair's 512 (exact), screening with the x0 interval; the eight
+// lanes of a group (p = lane mod 8) take draws p, p + 8, ... and the counts are summed over the group.
+// u1 is screened as an integer: draw i cannot count when (mix >> 11) > u1max, u1max >= ucut 2^53.
+__device__ int count_inside8(uint64_t seed, const double l[6], V3 m, double r, double r_alt, int *n_alt, int p) {
     // l = l00 l10 l20 l11 l21 l22
     const double l00 = l[0];
     double lo, hi;   // z0 interval where |m.x + l00 z0| < r
@@ -439,36 +534,62 @@ __device__ int count_inside(uint64_t seed, const double l[6], V3 m, double r, do
         lo = -1e300;
         hi = 1e300;
     }
-    const double a = all ? 0.0 : (lo > 0 ? lo : (hi < 0 ? -hi : 0.0));
-    const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u1 above this: |z0| < a
-    const float flo = (float)lo - 1e-3f, fhi = (float)hi + 1e-3f;
-    int cnt = 0, alt = 0;
     // |z0| <= 9.42 for every representable u1; with l00 = 0, x0 = m.x for every draw
     if ((l00 > 0 && (lo >= 9.5 || hi <= -9.5)) || (l00 <= 0 && !all)) {
         if (n_alt) *n_alt = 0;
         return 0;
     }
-    for (int i = 0; i < 512; i++) {
+    const double a = all ? 0.0 : (lo > 0 ? lo : (hi < 0 ? -hi : 0.0));
+    const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u1 above this: |z0| < a
+    const uint64_t u1max = ucut >= 1.0 ? ~0ull : (uint64_t)(ucut * 9007199254740992.0) + 1;
+    const float flo = (float)lo - 1e-3f, fhi = (float)hi + 1e-3f;
+    // |position estimate - position| <= 1e-3 (sum of |L|) when every z estimate is within 1e-3 of z
+    const double marg = 1e-3 * (fabs(l00) + fabs(l[1]) + fabs(l[2]) + fabs(l[3]) + fabs(l[4]) + fabs(l[5])) +
+                        1e-12 * fmax(r, r_alt);
+    int cnt = 0, alt = 0;
+    for (int i = p; i < 512; i += 8) {
         const uint64_t b = seed + (uint64_t)(6 * i) * GOLDEN_GAMMA;
-        const double u1 = to_unit(mix64(b + GOLDEN_GAMMA));
-        if (u1 > ucut) continue;
+        const uint64_t w1 = mix64(b + GOLDEN_GAMMA) >> 11;
+        if (w1 > u1max) continue;
+        const double u1 = (double)w1 * (1.0 / 9007199254740992.0);
         const double u2 = to_unit(mix64(b + 2 * GOLDEN_GAMMA));
-        if (!all) {
-            const float ze = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, u2);
-            if (ze < flo || ze > fhi) continue;
+        const float ze0 = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, u2);
+        if (!all && (ze0 < flo || ze0 > fhi)) continue;
+        // FP32 estimate of the whole draw: inside / outside decided unless within the margin of a radius
+        const double u3 = to_unit(mix64(b + 3 * GOLDEN_GAMMA)), u4 = to_unit(mix64(b + 4 * GOLDEN_GAMMA));
+        const double u5 = to_unit(mix64(b + 5 * GOLDEN_GAMMA)), u6 = to_unit(mix64(b + 6 * GOLDEN_GAMMA));
+        {
+            const double e0 = ze0;
+            const double e1 = z0_estimate(u3 <= 0.0 ? 1.0 / 18446744073709551616.0 : u3, u4);
+            const double e2 = z0_estimate(u5 <= 0.0 ? 1.0 / 18446744073709551616.0 : u5, u6);
+            const double xe = m.x + l00 * e0, ye = m.y + l[1] * e0 + l[3] * e1,
+                         ze = m.z + l[2] * e0 + l[4] * e1 + l[5] * e2;
+            const double re = sqrt(xe * xe + ye * ye + ze * ze);
+            const bool sure_r = re + marg < r || re - marg >= r, sure_a = re + marg < r_alt || re - marg >= r_alt;
+            if (sure_r && sure_a) {
+                cnt += re + marg < r;
+                alt += re + marg < r_alt;
+                continue;
+            }
         }
         const double z0 = box_muller(u1, u2);
-        const double z1 = box_muller(to_unit(mix64(b + 3 * GOLDEN_GAMMA)), to_unit(mix64(b + 4 * GOLDEN_GAMMA)));
-        const double z2 = box_muller(to_unit(mix64(b + 5 * GOLDEN_GAMMA)), to_unit(mix64(b + 6 * GOLDEN_GAMMA)));
+        const double z1 = box_muller(u3, u4);
+        const double z2 = box_muller(u5, u6);
         const double x = m.x + l00 * z0, y = m.y + l[1] * z0 + l[3] * z1, z = m.z + l[2] * z0 + l[4] * z1 + l[5] * z2;
         const double rr = sqrt(x * x + y * y + z * z);
         cnt += rr < r;
         alt += rr < r_alt;
     }
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        cnt += __shfl_xor(cnt, d, 64);
+        alt += __shfl_xor(alt, d, 64);
+    }
     if (n_alt) *n_alt = alt;
     return cnt;
 }
 
+// Eight lanes per kept pair (a group): every lane derives the pair's frame, the group counts the draws.
 __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ kept,
                                                      const unsigned long long *__restrict__ nkept, long kept_cap,
                                                      const double *__restrict__ strk, const double *__restrict__ ctrk,
@@ -480,7 +601,9 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
                                                      PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
                                                      long row_cap) {
     const long nk = min((long)*nkept, kept_cap);
-    for (long h = (long)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (long)gridDim.x * blockDim.x) {
+    const int p = threadIdx.x & 7;
+    const long g0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3, ng = ((long)gridDim.x * blockDim.x) >> 3;
+    for (long h = g0; h < nk; h += ng) {   // uniform within a group: its lanes shuffle with each other only
         const Kept kp = kept[h];
         const int s = kp.star, c = kp.cluster, kb = kp.kb;
         const int n = min(npts, cn[c]);
@@ -503,13 +626,14 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         const double r50 = r50s[c], rt = rts[c];
         int in50, intot;
         if (n50_min > 0) {
-            in50 = count_inside(seed, L, m, r50, r50, nullptr);
+            in50 = count_inside8(seed, L, m, r50, r50, nullptr, p);
             if (in50 < n50_min) continue;
-            intot = count_inside(seed, L, m, rt, rt, nullptr);
+            intot = count_inside8(seed, L, m, rt, rt, nullptr, p);
         } else {   // every kept pair is a row: count both, screening with the larger radius
-            const int n_big = count_inside(seed, L, m, fmax(rt, r50), rt < r50 ? rt : r50, &in50);
+            const int n_big = count_inside8(seed, L, m, fmax(rt, r50), rt < r50 ? rt : r50, &in50, p);
             if (rt < r50) { intot = in50; in50 = n_big; } else intot = n_big;
         }
+        if (p != 0) continue;
         PhxRow o;
         o.star = s; o.cluster = c; o.kb = kb; o.flags = sgroup[s];
         o.chi2 = kp.chi2; o.sb = kp.sb;
@@ -523,14 +647,6 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         const unsigned long long ro = atomicAdd(nrows, 1ull);
         if ((long)ro < row_cap) rows[ro] = o;
     }
-}
-
-// float4 tracks (position, lambda bound) from FP64 tracks [n][npts][12] (the exact-track variant).
-__global__ __launch_bounds__(256) void k_trkf(const double *__restrict__ trk, long npoints, float4 *__restrict__ trkf) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= npoints) return;
-    const double *p = trk + id * 12;
-    trkf[id] = make_float4((float)p[0], (float)p[1], (float)p[2], up(lam_bound(p + 6) * (1.0 + 1e-6)));
 }
 
 // Largest |z0_estimate - z0| over n (u1, u2) pairs drawn from splitmix64(seed): the margin check of the
@@ -551,116 +667,138 @@ __global__ void k_zero_counters(unsigned long long *c, int n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
 
+
 // ------------------------------------------------------------------------------------------- host side
 static int n_tiles(int npts) { return (npts + TILE - 1) / TILE; }
+static int ns_pad(int ns) { return (ns + SPW - 1) / SPW * SPW; }
 
 int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, const double *d_R2, int nc, int npts,
                       hipStream_t st) {
+    PHX_REQUIRE(n_tiles(npts) <= 64, "fast forms: tracks longer than 512 points (256 Myr) need the plain form");
     T.nc = nc;
     T.npts = npts;
     T.ntiles = n_tiles(npts);
-    T.chi2_built = -1.0;
+    const long ntiled = (long)nc * T.ntiles * TREC;
     if (ensure(T.mem[T_CTRK], T.bytes[T_CTRK], sizeof(double) * 12 * (size_t)nc * npts) ||
-        ensure(T.mem[T_CTRKF], T.bytes[T_CTRKF], sizeof(float4) * (size_t)nc * npts))
+        ensure(T.mem[T_CTILED], T.bytes[T_CTILED], sizeof(double) * (size_t)ntiled))
         return 1;
     hipLaunchKernelGGL(k_cluster_aos, dim3(blocks_for((long)nc * npts, 256)), dim3(256), 0, st, d_csoa, d_cn, nc, npts,
-                       (double *)T.mem[T_CTRK], (float4 *)T.mem[T_CTRKF]);
+                       (double *)T.mem[T_CTRK]);
+    hipLaunchKernelGGL(k_tiled, dim3(blocks_for(ntiled, 256)), dim3(256), 0, st, (const double *)T.mem[T_CTRK], d_cn,
+                       0, nc, npts, T.ntiles, (double *)T.mem[T_CTILED]);
     PHX_CK(hipGetLastError());
-    // sort clusters by tile count, longest first (length buckets of the tile scan)
-    std::vector<int> cn(nc), perm(nc), nt_sorted(nc);
+    // sort clusters by track length, longest first (length buckets of the tile scan)
+    std::vector<int> cn(nc), perm(nc), nt_sorted(nc), cn_sorted(nc);
     PHX_CK(hipMemcpyAsync(cn.data(), d_cn, 4 * (size_t)nc, hipMemcpyDeviceToHost, st));
     PHX_CK(hipStreamSynchronize(st));
     for (int i = 0; i < nc; i++) perm[i] = i;
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return cn[a] > cn[b]; });
-    for (int i = 0; i < nc; i++) nt_sorted[i] = n_tiles(cn[perm[i]]);
+    for (int i = 0; i < nc; i++) {
+        nt_sorted[i] = n_tiles(cn[perm[i]]);
+        cn_sorted[i] = cn[perm[i]];
+    }
     T.nc_pad = (nc + 63) / 64 * 64;
+    const size_t nrec = (size_t)T.ntiles * T.nc_pad;
     if (ensure(T.mem[T_CNT], T.bytes[T_CNT], 4 * (size_t)T.nc_pad) ||
-        ensure(T.mem[T_CPERM], T.bytes[T_CPERM], 4 * (size_t)T.nc_pad))
+        ensure(T.mem[T_CPERM], T.bytes[T_CPERM], 4 * (size_t)T.nc_pad) ||
+        ensure(T.mem[T_CNS], T.bytes[T_CNS], 4 * (size_t)T.nc_pad) ||
+        ensure(T.mem[T_CREC1], T.bytes[T_CREC1], sizeof(TileRec1) * nrec) ||
+        ensure(T.mem[T_CREC2], T.bytes[T_CREC2], sizeof(TileRec2) * nrec))
         return 1;
     nt_sorted.resize(T.nc_pad, 0);
+    cn_sorted.resize(T.nc_pad, 0);
     perm.resize(T.nc_pad, 0);
     PHX_CK(hipMemcpyAsync(T.mem[T_CNT], nt_sorted.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
+    PHX_CK(hipMemcpyAsync(T.mem[T_CNS], cn_sorted.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
     PHX_CK(hipMemcpyAsync(T.mem[T_CPERM], perm.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_cluster_recs, dim3(blocks_for((long)nrec, 256)), dim3(256), 0, st,
+                       (const double *)T.mem[T_CTRK], d_cn, d_R2, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
+                       T.ntiles, (TileRec1 *)T.mem[T_CREC1], (TileRec2 *)T.mem[T_CREC2]);
+    PHX_CK(hipGetLastError());
     PHX_CK(hipStreamSynchronize(st));
     T.d_cn = d_cn;
     T.d_R2 = d_R2;
     return 0;
 }
 
-static int build_cluster_tiles(FastTables &T, double chi2_max, hipStream_t st) {
-    if (T.chi2_built == chi2_max) return 0;
-    const size_t nt = (size_t)T.ntiles * T.nc_pad;
-    if (ensure(T.mem[T_CTLO], T.bytes[T_CTLO], sizeof(float4) * nt) ||
-        ensure(T.mem[T_CTHI], T.bytes[T_CTHI], sizeof(float4) * nt))
-        return 1;
-    hipLaunchKernelGGL(k_cluster_tiles, dim3(blocks_for((long)nt, 256)), dim3(256), 0, st,
-                       (const double *)T.mem[T_CTRK], (const float4 *)T.mem[T_CTRKF], T.d_cn, T.d_R2,
-                       (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, T.npts, T.ntiles, (float)sqrt(chi2_max),
-                       (float4 *)T.mem[T_CTLO], (float4 *)T.mem[T_CTHI]);
-    PHX_CK(hipGetLastError());
-    T.chi2_built = chi2_max;
-    return 0;
-}
-
 static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st) {
+    (void)st;
     const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     const long npairs = (long)ns * a.nc;
-    if (W.cand_cap == 0) W.cand_cap = std::max(1L << 20, npairs / 5);
+    if (W.cand_cap == 0) W.cand_cap = std::max(1L << 20, npairs / 6);
     if (W.kept_cap == 0) W.kept_cap = std::max(1L << 18, npairs / 16);
     if (W.row_cap == 0) W.row_cap = std::max(1L << 16, W.kept_cap / 8);
+    if (W.item_cap == 0) W.item_cap = std::max(1L << 22, npairs);
     if (ensure(W.mem[W_PS0], W.bytes[W_PS0], 48 * (size_t)ns) || ensure(W.mem[W_SIG], W.bytes[W_SIG], 288 * (size_t)ns) ||
         ensure(W.mem[W_TRK], W.bytes[W_TRK], sizeof(double) * 12 * (size_t)ns * npts) ||
-        ensure(W.mem[W_TRKF], W.bytes[W_TRKF], sizeof(float4) * (size_t)ns * npts) ||
-        ensure(W.mem[W_STILE], W.bytes[W_STILE], 2 * sizeof(float4) * (size_t)ns * ntl) ||
+        ensure(W.mem[W_TILED], W.bytes[W_TILED], sizeof(double) * TREC * (size_t)ns * ntl) ||
+        ensure(W.mem[W_SREC1], W.bytes[W_SREC1], sizeof(TileRec1) * (size_t)ns_pad(ns) * ntl) ||
+        ensure(W.mem[W_SREC2], W.bytes[W_SREC2], sizeof(TileRec2) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_CAND], W.bytes[W_CAND], sizeof(Cand) * (size_t)W.cand_cap) ||
         ensure(W.mem[W_KEPT], W.bytes[W_KEPT], sizeof(Kept) * (size_t)W.kept_cap) ||
         ensure(W.mem[W_ROWS], W.bytes[W_ROWS], sizeof(PhxRow) * (size_t)W.row_cap) ||
-        ensure(W.mem[W_CNT], W.bytes[W_CNT], 4 * sizeof(unsigned long long)))
+        ensure(W.mem[W_ITEMS], W.bytes[W_ITEMS], sizeof(Item) * (size_t)W.item_cap) ||
+        ensure(W.mem[W_IRES], W.bytes[W_IRES], sizeof(IRes) * (size_t)W.item_cap) ||
+        ensure(W.mem[W_ISTART], W.bytes[W_ISTART], sizeof(long) * (size_t)W.cand_cap) ||
+
+        ensure(W.mem[W_CNT], W.bytes[W_CNT], C_N * sizeof(unsigned long long)))
         return 1;
-    return build_cluster_tiles(T, a.chi2_max, st);
+    return 0;
 }
 
 // The pass's launches only (no allocation, no host synchronisation): capturable into a hipGraph.
 static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
     const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     unsigned long long *cnt = (unsigned long long *)W.mem[W_CNT];
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cnt, 4);
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cnt, C_N);
     PHX_CK(hipGetLastError());
     double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
-    float4 *trkf = (float4 *)W.mem[W_TRKF];
-    float4 *stlo = (float4 *)W.mem[W_STILE], *sthi = stlo + (size_t)ns * ntl;
+    TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
+    TileRec2 *srec2 = (TileRec2 *)W.mem[W_SREC2];
     if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) return 1;
     if (mark) mark(mark_ctx, "draws");
     if (a.exact_tracks) {   // the plain FP64 tracks: rows identical to the plain form's
         if (launch_lin_plain(ps0, sig, nullptr, trk, a.P, PH_LIN_EPS_POS, PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB,
                              PH_LIN_SUB, 0, npts, npts, ns, 0, nullptr, st))
             return 1;
-        hipLaunchKernelGGL(k_trkf, dim3(blocks_for((long)ns * npts, 256)), dim3(256), 0, st, trk, (long)ns * npts, trkf);
     } else {
-        hipLaunchKernelGGL(k_lin_fast, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk, trkf, a.P,
+        hipLaunchKernelGGL(k_lin_fast, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
                            make_potf(a.P), npts, ns);
     }
+    hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts, ns,
+                       npts, ntl, (double *)W.mem[W_TILED]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
-    hipLaunchKernelGGL(k_star_tiles, dim3(blocks_for((long)ns * ntl, 256)), dim3(256), 0, st, trk, trkf, npts, ntl, ns,
-                       (float)sqrt(a.chi2_max), stlo, sthi);
+    hipLaunchKernelGGL(k_star_recs, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts, ntl, ns,
+                       ns_pad(ns), srec1, srec2);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tiles");
+    const double chi2c = a.chi2_max * (1.0 + CUT_REL);
     hipLaunchKernelGGL(k_scan_tiles, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
-                       (const float4 *)T.mem[T_CTLO], (const float4 *)T.mem[T_CTHI], (const int *)T.mem[T_CNT],
-                       (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, stlo, sthi, ns, ntl, (Cand *)W.mem[W_CAND], cnt,
-                       W.cand_cap);
+                       (const TileRec1 *)T.mem[T_CREC1], (const TileRec2 *)T.mem[T_CREC2], (const int *)T.mem[T_CNT],
+                       (const int *)T.mem[T_CNS], (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, srec1, srec2, ns, ntl,
+                       up(sqrt(chi2c)), chi2c, (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "scan");
-    hipLaunchKernelGGL(k_refine, dim3(4096), dim3(256), 0, st, (const Cand *)W.mem[W_CAND], cnt, W.cand_cap, trk, trkf,
-                       (const double *)T.mem[T_CTRK], (const float4 *)T.mem[T_CTRKF], a.cn, a.R2, npts, a.chi2_max,
-                       (Kept *)W.mem[W_KEPT], cnt + 1, W.kept_cap);
+    const Cand *cand = (const Cand *)W.mem[W_CAND];
+    hipLaunchKernelGGL(k_expand, dim3(2048), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap, (Item *)W.mem[W_ITEMS],
+                       cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART]);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_items, dim3(16384), dim3(256), 0, st, (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
+                       (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
+                       (IRes *)W.mem[W_IRES]);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce, dim3(4096), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap,
+                       (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES], cnt + C_ITEMS, W.item_cap,
+                       (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
+                       a.chi2_max, (Kept *)W.mem[W_KEPT],
+                       cnt + C_KEPT, W.kept_cap);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "refine");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;
-    hipLaunchKernelGGL(k_finish_fast, dim3(4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + 1, W.kept_cap,
+    hipLaunchKernelGGL(k_finish_fast, dim3(4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap,
                        trk, (const double *)T.mem[T_CTRK], a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min,
-                       (PhxRow *)W.mem[W_ROWS], cnt + 2, W.row_cap);
+                       (PhxRow *)W.mem[W_ROWS], cnt + C_ROWS, W.row_cap);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "finish");
     return 0;
@@ -689,7 +827,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt};
     for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
     for (int i = 0; i < 16; i++) { mix((uint64_t)(uintptr_t)W.mem[i]); mix((uint64_t)(uintptr_t)T.mem[i]); }
-    mix((uint64_t)W.cand_cap); mix((uint64_t)W.kept_cap); mix((uint64_t)W.row_cap);
+    mix((uint64_t)W.cand_cap); mix((uint64_t)W.kept_cap); mix((uint64_t)W.row_cap); mix((uint64_t)W.item_cap);
     if (!W.gexec || W.gkey != key) {
         graph_drop(W);
         hipGraph_t g = nullptr;
@@ -711,16 +849,18 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
 }
 
 int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
-    unsigned long long c[3];
+    unsigned long long c[C_N];
     PHX_CK(hipMemcpyAsync(c, W.mem[W_CNT], sizeof c, hipMemcpyDeviceToHost, st));
     PHX_CK(hipStreamSynchronize(st));
-    W.last_cand = (long)c[0];
+    W.last_cand = (long)c[C_CAND];
+    W.last_items = (long)c[C_ITEMS];
     *grow = 0;
-    if ((long)c[0] > W.cand_cap) { W.cand_cap = (long)c[0] + (long)c[0] / 4 + 4096; *grow = 1; }
-    if ((long)c[1] > W.kept_cap) { W.kept_cap = (long)c[1] + (long)c[1] / 4 + 4096; *grow = 1; }
-    if ((long)c[2] > W.row_cap) { W.row_cap = (long)c[2] + (long)c[2] / 4 + 4096; *grow = 1; }
-    h[0] = (long)c[1];
-    h[1] = (long)c[2];
+    if ((long)c[C_CAND] > W.cand_cap) { W.cand_cap = (long)c[C_CAND] + (long)c[C_CAND] / 4 + 4096; *grow = 1; }
+    if ((long)c[C_ITEMS] > W.item_cap) { W.item_cap = (long)c[C_ITEMS] + (long)c[C_ITEMS] / 4 + 4096; *grow = 1; }
+    if ((long)c[C_KEPT] > W.kept_cap) { W.kept_cap = (long)c[C_KEPT] + (long)c[C_KEPT] / 4 + 4096; *grow = 1; }
+    if ((long)c[C_ROWS] > W.row_cap) { W.row_cap = (long)c[C_ROWS] + (long)c[C_ROWS] / 4 + 4096; *grow = 1; }
+    h[0] = (long)c[C_KEPT];
+    h[1] = (long)c[C_ROWS];
     return 0;
 }
 

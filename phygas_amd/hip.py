@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
         "phx_load_clusters": (_I, [_P, _P, _I, _I, _D]), "phx_load_stars": (_I, [_P, _P, _I, _I]),
         "phx_clear_stars": (_I, [_P]), "phx_screen": (_I, [_P, _I, _D, _I, _D]),
         "phx_rows": (_I, [_P, _P, _L, C.POINTER(_L)]), "phx_kept": (_I, [_P, C.POINTER(_L)]),
+        "phx_candidates": (_I, [_P, C.POINTER(_L)]),
         "phx_stage_times": (_I, [_P, _P, _P, _I]), "phx_set_timing": (None, [_P, _I]),
         "phx_cluster_tracks": (_I, [_P, _P, _P, _P, _P, _I]), "phx_star_tracks": (_I, [_P, _I, _I, _P, _I]),
         "phx_chi2_matrix": (_I, [_P, _P, _P, _P]), "phx_npts": (_I, [_P]),
@@ -177,6 +178,13 @@ class Engine:
     def kept(self) -> int:
         n = _L(0)
         _check(self._L.phx_kept(self.h, C.byref(n)), "phx_kept")
+        return n.value
+
+    def candidates(self) -> int:
+        """Pairs the last pass evaluated point by point (all pairs for PLAIN; the tile scan's survivors for the
+        fast forms)."""
+        n = _L(0)
+        _check(self._L.phx_candidates(self.h, C.byref(n)), "phx_candidates")
         return n.value
 
     def sync(self):

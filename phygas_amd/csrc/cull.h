@@ -11,7 +11,7 @@
 //   a, b   least-squares line p'_k ~ a + b (k - tc), tc = 8j + 4, through the rotated positions
 //   e      largest distance of a rotated position from that line (the tube radius)
 //   U      3x3 matrix with U >= R_k S_k R_k^T for every point of the tile (Loewner order): the rotated
-//          covariance of the tile's last point plus eta I, eta >= the largest Gershgorin bound of
+//          covariance of the tile's last point plus eta I, eta >= the largest eigenvalue of
 //          R_k S_k R_k^T - R_kl S_kl R_kl^T over the tile; clusters add the pair's kernel R^2 I
 //   lam    upper bound of the largest eigenvalue of U (the smaller of trace and Gershgorin bound)
 //
@@ -72,6 +72,29 @@ PHX_HDF double lam_bound(const double *S) {
                  g2 = S[5] + fabs(S[2]) + fabs(S[4]);
     const double g = fmax(g0, fmax(g1, g2));
     return fmax(fmin(tr, g), 0.0);
+}
+
+// Upper bound of the largest eigenvalue of a symmetric 3x3 (xx xy xz yy yz zz), possibly indefinite: the
+// closed form (trigonometric solution of the characteristic cubic) plus 1e-10 of the matrix's scale for its
+// rounding, and never above the Gershgorin bound.
+PHX_HDF double sym3_lmax_bound(const double *d) {
+    const double g0 = d[0] + fabs(d[1]) + fabs(d[2]), g1 = d[3] + fabs(d[1]) + fabs(d[4]),
+                 g2 = d[5] + fabs(d[2]) + fabs(d[4]);
+    const double gersh = fmax(g0, fmax(g1, g2));
+    const double p1 = d[1] * d[1] + d[2] * d[2] + d[4] * d[4];
+    const double q = (d[0] + d[3] + d[5]) / 3.0;
+    const double a = d[0] - q, b = d[3] - q, c = d[5] - q;
+    const double p2 = a * a + b * b + c * c + 2.0 * p1;
+    const double pp = sqrt(p2 / 6.0);
+    double lmax = q;
+    if (pp > 0.0) {
+        // det((D - qI) / pp) / 2
+        const double r = (a * (b * c - d[4] * d[4]) - d[1] * (d[1] * c - d[4] * d[2]) + d[2] * (d[1] * d[4] - b * d[2])) /
+                         (2.0 * pp * pp * pp);
+        const double phi = acos(fmin(fmax(r, -1.0), 1.0)) / 3.0;
+        lmax = q + 2.0 * pp * cos(phi);
+    }
+    return fmin(gersh, lmax + 1e-10 * (fabs(q) + 2.0 * pp));
 }
 
 // R_k S R_k^T of a symmetric 3x3 (xx xy xz yy yz zz), R_k the rotation by OMEGA_PT k about z.
@@ -147,9 +170,7 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
             double d[6];
 #pragma unroll
             for (int u = 0; u < 6; u++) d[u] = q[u] - ref[u];
-            const double g0 = d[0] + fabs(d[1]) + fabs(d[2]), g1 = d[3] + fabs(d[1]) + fabs(d[4]),
-                         g2 = d[5] + fabs(d[2]) + fabs(d[4]);
-            eta = fmax(eta, fmax(g0, fmax(g1, g2)));
+            eta = fmax(eta, sym3_lmax_bound(d));
             trmax = fmax(trmax, q[0] + q[3] + q[5]);
         }
     }

@@ -291,21 +291,25 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
                 for (int i = 0; i < SPW; i++) mask[i] |= (unsigned long long)((l2 >> i) & 1u) << j;
             }
         }
+        // the task's candidates take one range of the list (one atomic per task)
+        unsigned total = 0;
+#pragma unroll
+        for (int i = 0; i < SPW; i++) total += (unsigned)__popcll(__ballot(cs < nc && s0 + i < ns && mask[i] != 0ull));
+        if (total == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(ncand, (unsigned long long)total);
+        base = __shfl(base, 0, 64);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int cl = cs < nc ? perm[cs] : 0;
 #pragma unroll
         for (int i = 0; i < SPW; i++) {
-            if (s0 + i < ns) {   // wave-uniform
-                const bool hit = cs < nc && mask[i] != 0ull;
-                const unsigned long long b = __ballot(hit);
-                if (b != 0ull) {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(ncand, (unsigned long long)__popcll(b));
-                    base = __shfl(base, 0, 64);
-                    if (hit) {
-                        const unsigned long long o = base + __popcll(b & ((1ull << lane) - 1ull));
-                        if ((long)o < cap) cand[o] = Cand{s0 + i, perm[cs], mask[i]};
-                    }
-                }
+            const bool hit = cs < nc && s0 + i < ns && mask[i] != 0ull;
+            const unsigned long long b = __ballot(hit);
+            if (hit) {
+                const unsigned long long o = base + __popcll(b & below);
+                if ((long)o < cap) cand[o] = Cand{s0 + i, cl, mask[i]};
             }
+            base += __popcll(b);
         }
     }
 }

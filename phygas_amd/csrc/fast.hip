@@ -17,7 +17,7 @@
 //                    Mahalanobis forms per point, strict-< first-index minimum)
 //      k_reduce      one lane per candidate: minimum over its items; pairs with chi2 < chi2_max are appended
 //                    to the kept list
-//   6. k_finish_fast one lane per kept pair: the 512 Box-Muller draws of the pair's seed, screened draw
+//   6. k_finish_fast eight lanes per kept pair: the 512 Box-Muller draws of the pair's seed, screened draw
 //                    by draw (below), exact counts; rows with p_r50 >= p_min appended
 //
 // Exactness of the cut: a tile dropped by the scan holds no point with chi2 < chi2_max (cull.h), and the

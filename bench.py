@@ -26,16 +26,34 @@ sys.path.insert(0, ROOT)
 from phygas_amd import hip  # noqa: E402
 from phygas_amd import workload as wl  # noqa: E402
 
-PEAK_FP64_VALU = 78.6     # TFLOP/s, MI355X vector FP64
-PEAK_FP32_VALU = 157.3    # TFLOP/s, MI355X vector FP32
+# MI355X published peaks.  The vector units run FP64 and unpacked FP32 at the same rate (78.6 TFLOP/s, FMA = 2
+# FLOP); this build disables packed FP32, so 78.6 TFLOP/s bounds both the FP32 and the FP64 work of a kernel.
+PEAK_VALU = 78.6          # TFLOP/s
+PEAK_HBM = 8000.0         # GB/s
 METRIC = "star-cluster pairs/s, chosen vs plain GPU at |dp_r50|<=1e-4; p50/p99 latency"
 
+# Algorithmic work per unit (FMA = 2 FLOP, add/sub/mul/div/sqrt/compare = 1), counted from the source:
+PLAIN_POINT_FLOP = 126.0   # k_chi2_plain, one interior pair-point (56 at a track's last point)
+PLAIN_LAST_FLOP = 56.0
+SCAN_L1_FLOP = 76.0        # cull.h l1_pass, one (pair, tile) test, FP32
+SCAN_L2_FLOP = 177.0       # cull.h l2_pass, one (pair, tile) certificate, FP64
+ITEM_BYTES = 2 * 75 * 8    # k_items, one (pair, tile) item: the star's and the cluster's tiled records
 
-def scan_flops(cn: np.ndarray, n_stars: int) -> float:
-    """Algorithmic FP64 work of the full scan: 126 FLOP per interior pair-point, 56 at a track's last point
-    (add/sub/mul/div/sqrt = 1 FLOP), summed over every cluster, times the stars."""
-    per_star = float(np.sum(126.0 * (cn - 1) + 56.0))
+
+def plain_scan_flops(cn: np.ndarray, n_stars: int) -> float:
+    """Algorithmic FP64 work of the plain full scan, summed over every cluster, times the stars."""
+    per_star = float(np.sum(PLAIN_POINT_FLOP * (cn - 1) + PLAIN_LAST_FLOP))
     return per_star * n_stars
+
+
+def pmc_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of a kernel on this workload from the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
+    f = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return float(json.load(fh)["traffic_bytes"])
 
 
 def cpu_baseline(clusters, stars, threads: int) -> dict | None:
@@ -91,6 +109,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-rate", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true", help="default launch grids instead of the autotuner's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the screening shards by star with no "
@@ -113,18 +132,13 @@ def main() -> int:
     e.load_stars(control, 1)
 
     form = hip.FAST if args.form == "fast" else hip.PLAIN
-    try:
-        ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
-    except hip.PhygasError as err:
-        if form == hip.FAST and "not built yet" in str(err):
-            print("fast form not available yet; timing the plain form", file=sys.stderr)
-            form = hip.PLAIN
-            ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
-        else:
-            raise
+    tune = None
+    if form == hip.FAST and not args.no_tune:
+        tune = e.autotune(hip.FAST, chi2_max, 64, p_min)   # launch grids, chosen before the timed steps
+    ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
     rows = e.rows()
     kept = e.kept()
-    candidates = e.candidates()
+    stats = e.pass_stats()
     result = {"form": "fast" if form == hip.FAST else "plain", "ms": ms, "stages": stages, "rows": len(rows),
               "kept": kept}
 
@@ -141,21 +155,30 @@ def main() -> int:
                  "rows_only_fast": len(set(a) - set(b)), "rows_only_plain": len(set(b) - set(a)),
                  "max_abs_dp_r50": dp, "pairs_dp_r50_above_1e-4": nbad}
 
-    # roofline of the dominant kernel: the scan of the plain form (FP64 VALU); for the fast form, its scan
+    # roofline of the dominant kernel (largest stage of the timed pass)
     if form == hip.PLAIN:
         k_ms = stages.get("scan", float("nan"))
-        flops = scan_flops(cn, n_stars)
+        flops = plain_scan_flops(cn, n_stars)
         achieved = flops / (k_ms / 1000.0) / 1e12
-        roof = {"bound": "valu", "kernel": "k_chi2_plain (full FP64 scan)", "achieved": achieved,
-                "peak": PEAK_FP64_VALU, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU, "traffic": None,
-                "work_per_launch_flop": flops, "avg_launch_ms": k_ms}
+        roof = {"bound": "valu", "kernel": "k_chi2_plain", "achieved": achieved, "peak": PEAK_VALU,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_chi2_plain"),
+                "work_per_launch": flops, "avg_launch_ms": k_ms}
     else:
-        k_ms = stages.get("scan", float("nan"))
-        flops = scan_flops(cn, n_stars)
-        achieved = flops / (k_ms / 1000.0) / 1e12
-        roof = {"bound": "valu", "kernel": "fast scan (effective: full-scan FLOP count / time)",
-                "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU,
-                "traffic": None, "work_per_launch_flop": flops, "avg_launch_ms": k_ms}
+        # the pair scan (every pair of the unit) or the point evaluation of the tiles it keeps, whichever is longer
+        name = max(("scan", "items"), key=lambda k: stages.get(k, 0.0))
+        k_ms = stages.get(name, float("nan"))
+        if name == "items":
+            work = stats["items"] * ITEM_BYTES
+            achieved = work / (k_ms / 1000.0) / 1e9
+            roof = {"bound": "hbm", "kernel": "k_items", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM, "traffic": pmc_traffic("k_items"), "work_per_launch": work,
+                    "avg_launch_ms": k_ms}
+        else:
+            work = stats["l1_tests"] * SCAN_L1_FLOP + stats["l2_tests"] * SCAN_L2_FLOP
+            achieved = work / (k_ms / 1000.0) / 1e12
+            roof = {"bound": "valu", "kernel": "k_scan_tiles", "achieved": achieved, "peak": PEAK_VALU,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_scan_tiles"),
+                    "work_per_launch": work, "avg_launch_ms": k_ms}
 
     e.close()
     serving = None
@@ -185,7 +208,7 @@ def main() -> int:
                    len(control), "clusters": len(clusters), "pairs_per_step": pairs, "chi2_max": chi2_max,
                    "n_samp": 64, "r_kernel": 2.0, "p_min": p_min, "form": result["form"],
                    "cluster_tables_s": tables_s},
-        "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "candidate_pairs": candidates,
+        "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "pass_stats": stats, "autotune": tune,
         "roofline": roof, "cpu_baseline": cpu,
     }
     if plain:

@@ -92,6 +92,10 @@ int phx_kept(PhxCtx *ctx, long *n);
 /* Pairs the last pass evaluated point by point: every pair for the plain form, the pairs left by the tile
  * scan for the fast forms. */
 int phx_candidates(PhxCtx *ctx, long *n);
+/* Work counts of the last pass into st[0 .. min(cap, 6)): (pair, tile) tests of the scan's first level,
+ * certificate tests of its second level, candidate pairs, point-evaluation items, kept pairs, rows.  For the
+ * plain form the first and third entries are the pair count and the second and fourth 0. */
+int phx_pass_stats(PhxCtx *ctx, long *st, int cap);
 /* Durations (ms, HIP events on the context's stream) of the last pass's stages; returns the count, names
  * into names[] (static strings). */
 int phx_stage_times(PhxCtx *ctx, double *ms, const char **names, int cap);
@@ -100,6 +104,11 @@ void phx_set_timing(PhxCtx *ctx, int on);
  * same (a pass with timing on runs eagerly, its stage events between the kernels). */
 void phx_set_graphs(PhxCtx *ctx, int on);
 int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
+
+/* Launch grids of the fast pass's tile scan, item evaluation, kept-pair reduction and finish, chosen by
+ * timing each on the loaded batch (results do not depend on them); grids[4] and ms[4] receive the choices
+ * and their stage times.  No counterpart in the reference. */
+int phx_autotune(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms);
 
 /* Parity access to intermediate device results of the last plain pass / the cluster tables. */
 int phx_cluster_tracks(PhxCtx *ctx, int *n, double *pos, double *vel, double *S, int npts_cap);

@@ -399,6 +399,67 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
     return last_counts(c, n, &r);
 }
 
+// Launch grids of the fast pass chosen by timing each stage on the loaded batch: for the tile scan, the item
+// evaluation, the kept-pair reduction and the finish, every grid in a short list is tried (the others held
+// at their current values) and the fastest kept.  Results do not depend on the grids.
+extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms) {
+    PHX_REQUIRE(form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "the autotuner tunes the fast forms");
+    PHX_REQUIRE(!c->st_h.empty(), "load stars first");
+    if (phx_screen(c, form, chi2_max, n_samp, p_min)) return 1;
+    long kept, nr;
+    if (last_counts(c, &kept, &nr)) return 1;   // sizes the buffers
+    const bool timing = c->timing;
+    c->timing = true;
+    struct Knob { int *v; const char *stage; int opts[5]; };
+    Knob knobs[4] = {{&c->fw.scan_blocks, "scan", {1024, 2048, 4096, 8192, 0}},
+                     {&c->fw.items_blocks, "items", {4096, 8192, 16384, 32768, 0}},
+                     {&c->fw.reduce_blocks, "reduce", {1024, 2048, 4096, 8192, 0}},
+                     {&c->fw.finish_blocks, "finish", {1024, 2048, 4096, 8192, 0}}};
+    int rc = 0;
+    for (int q = 0; q < 4 && rc == 0; q++) {
+        int best = *knobs[q].v;
+        double best_ms = 1e300;
+        for (int o = 0; o < 5 && knobs[q].opts[o] > 0; o++) {
+            *knobs[q].v = knobs[q].opts[o];
+            double t = 0.0;
+            for (int rep = 0; rep < 2 && rc == 0; rep++) {   // best of two passes
+                if ((rc = phx_screen(c, form, chi2_max, n_samp, p_min)) != 0) break;
+                double st[MAX_STAGES];
+                const char *nm[MAX_STAGES];
+                const int n = phx_stage_times(c, st, nm, MAX_STAGES);
+                for (int i = 0; i < n; i++)
+                    if (strcmp(nm[i], knobs[q].stage) == 0) t = rep == 0 ? st[i] : std::min(t, st[i]);
+            }
+            if (rc == 0 && t < best_ms) { best_ms = t; best = knobs[q].opts[o]; }
+        }
+        *knobs[q].v = best;
+        if (grids) grids[q] = best;
+        if (ms) ms[q] = best_ms;
+    }
+    c->timing = timing;
+    return rc;
+}
+
+extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
+    long kept, nr;
+    if (last_counts(c, &kept, &nr)) return 1;
+    long v[6] = {0, 0, 0, 0, kept, nr};
+    const long ns = (long)c->st_h.size();
+    if (c->last_form == PHX_FORM_PLAIN) {
+        v[0] = ns * c->nc;   // every pair is scanned point by point
+        v[2] = ns * c->nc;
+    } else {
+        long tiles = 0;
+        for (int i = 0; i < c->nc; i++) tiles += (c->cn_h[i] + 7) / 8;
+        v[0] = ns * tiles;
+        v[1] = c->fw.last_l2;
+        v[2] = c->fw.last_cand;
+        v[3] = c->fw.last_items;
+    }
+    for (int i = 0; i < cap && i < 6; i++) st[i] = v[i];
+    return 0;
+}
+
 extern "C" int phx_candidates(PhxCtx *c, long *n) {
     long kept, r;
     if (last_counts(c, &kept, &r)) return 1;

@@ -56,8 +56,8 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 // Work buffers (FastWork::mem slots)
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
        W_TILED, W_NSLOT };
-// Counters (W_CNT): candidates, kept pairs, rows, items, scan task
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_N };
+// Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_N };
 // Table buffers (FastTables::mem slots)
 enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_NSLOT };
 
@@ -69,6 +69,13 @@ static int ensure(void *&p, size_t &have, size_t need) {
     PHX_CK(hipMalloc(&p, need ? need : 8));
     have = need ? need : 8;
     return 0;
+}
+
+// Wave-local ordering of LDS traffic between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ------------------------------------------------------------------------------------------- FP64 nominal
@@ -232,10 +239,15 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
 }
 
 // ------------------------------------------------------------------------------------------- tile scan
-// Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) from
-// a counter until none is left, lane = cluster.  Per tile j: level 1 for every star of the block (the star's record is wave-uniform and
-// read through scalar loads), then level 2 for the lane's level-1 survivors.  Pairs with any tile left are
-// appended with the bit mask of those tiles.
+// Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) from a
+// counter until none is left, lane = cluster.  Per tile j:
+//   level 1  every star of the block against the lane's cluster (the star's record is wave-uniform and read
+//            through scalar loads);
+//   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
+//            evaluated 64 at a time, one per lane, with the cluster records of the tile staged in LDS; the
+//            verdicts return to the cluster's lane as star bits.
+// Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64 tiles).
+template <typename MaskT>
 __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__ crec1,
                                                     const TileRec2 *__restrict__ crec2,
                                                     const int *__restrict__ ntc_sorted,
@@ -244,8 +256,15 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
                                                     const TileRec2 *__restrict__ srec2, int ns, int ntl,
                                                     float kchi, double chi2c, Cand *__restrict__ cand,
                                                     unsigned long long *__restrict__ ncand, long cap,
-                                                    unsigned long long *__restrict__ next_task) {
-    const int lane = threadIdx.x & 63;
+                                                    unsigned long long *__restrict__ next_task,
+                                                    unsigned long long *__restrict__ n_l2_total) {
+    __shared__ unsigned short q_s[4][SPW * 64];
+    __shared__ TileRec2 c2_s[4][64];
+    __shared__ float t1_s[4][64];
+    __shared__ unsigned l2_s[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned short *q = q_s[w];
+    TileRec2 *c2s = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
     const int ntasks = ncb * nsb;
     for (;;) {
@@ -260,9 +279,10 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
         const int ncl = cs < nc ? cn_sorted[cs] : 0;
         const int ntmax = __builtin_amdgcn_readfirstlane(nt);   // sorted descending: lane 0 has the longest
         const int s0 = sb * SPW;
-        unsigned long long mask[SPW];
+        unsigned n_l2 = 0;   // level-2 tests of this task (wave-uniform)
+        MaskT mask[SPW];
 #pragma unroll
-        for (int i = 0; i < SPW; i++) mask[i] = 0ull;
+        for (int i = 0; i < SPW; i++) mask[i] = 0;
         for (int j = 0; j < ntmax; j++) {
             const bool cv = j < nt;
             const TileRec1 c1 = crec1[(size_t)j * nc_pad + cs];
@@ -278,23 +298,48 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
                     if (l1_pass(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
             }
             if (!cv) l1 = 0u;
-            if (l1) {
-                const TileRec2 c2 = crec2[(size_t)j * nc_pad + cs];
-                unsigned l2 = 0u;
-                while (l1) {
-                    const int i = __builtin_ctz(l1);
-                    l1 &= l1 - 1u;
-                    const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
-                    if (l2_pass(s2, c2, (double)t1, chi2c)) l2 |= 1u << i;
-                }
+            // queue the wavefront's level-1 survivors: exclusive prefix of the per-lane counts
+            const unsigned cnt = (unsigned)__popc(l1);
+            unsigned x = cnt;
 #pragma unroll
-                for (int i = 0; i < SPW; i++) mask[i] |= (unsigned long long)((l2 >> i) & 1u) << j;
+            for (int d = 1; d < 64; d <<= 1) {
+                const unsigned y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
             }
+            const int nq = __builtin_amdgcn_readfirstlane(__shfl(x, 63, 64));
+            if (nq == 0) continue;
+            n_l2 += nq;
+            {
+                unsigned o = x - cnt, b = l1;
+                while (b) {
+                    const int i = __builtin_ctz(b);
+                    b &= b - 1u;
+                    q[o++] = (unsigned short)(i << 6 | lane);
+                }
+            }
+            if (cnt) c2s[lane] = crec2[(size_t)j * nc_pad + cs];
+            t1_s[w][lane] = t1;
+            l2_s[w][lane] = 0u;
+            wave_sync();
+            for (int e0 = 0; e0 < nq; e0 += 64) {   // wave-uniform
+                const int e = e0 + lane;
+                if (e < nq) {
+                    const int ent = q[e], i = ent >> 6, c = ent & 63;
+                    const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
+                    if (l2_pass(s2, c2s[c], (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
+                }
+            }
+            wave_sync();
+            const unsigned l2 = l2_s[w][lane];
+#pragma unroll
+            for (int i = 0; i < SPW; i++) mask[i] |= (MaskT)((l2 >> i) & 1u) << j;
+            wave_sync();   // q / c2s / l2_s are rewritten by the next tile
         }
+        if (lane == 0 && n_l2) atomicAdd(n_l2_total, (unsigned long long)n_l2);
         // the task's candidates take one range of the list (one atomic per task)
         unsigned total = 0;
 #pragma unroll
-        for (int i = 0; i < SPW; i++) total += (unsigned)__popcll(__ballot(cs < nc && s0 + i < ns && mask[i] != 0ull));
+        for (int i = 0; i < SPW; i++) total += (unsigned)__popcll(__ballot(cs < nc && s0 + i < ns && mask[i] != 0));
         if (total == 0) continue;
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(ncand, (unsigned long long)total);
@@ -303,11 +348,11 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
         const int cl = cs < nc ? perm[cs] : 0;
 #pragma unroll
         for (int i = 0; i < SPW; i++) {
-            const bool hit = cs < nc && s0 + i < ns && mask[i] != 0ull;
+            const bool hit = cs < nc && s0 + i < ns && mask[i] != 0;
             const unsigned long long b = __ballot(hit);
             if (hit) {
                 const unsigned long long o = base + __popcll(b & below);
-                if ((long)o < cap) cand[o] = Cand{s0 + i, cl, mask[i]};
+                if ((long)o < cap) cand[o] = Cand{s0 + i, cl, (unsigned long long)mask[i]};
             }
             base += __popcll(b);
         }
@@ -778,29 +823,33 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
+    auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
+    hipLaunchKernelGGL(scan, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
                        (const TileRec1 *)T.mem[T_CREC1], (const TileRec2 *)T.mem[T_CREC2], (const int *)T.mem[T_CNT],
                        (const int *)T.mem[T_CNS], (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, srec1, srec2, ns, ntl,
-                       up(sqrt(chi2c)), chi2c, (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK);
+                       up(sqrt(chi2c)), chi2c, (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK,
+                       cnt + C_L2);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "scan");
     const Cand *cand = (const Cand *)W.mem[W_CAND];
     hipLaunchKernelGGL(k_expand, dim3(2048), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap, (Item *)W.mem[W_ITEMS],
                        cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART]);
     PHX_CK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(16384), dim3(256), 0, st, (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
+    if (mark) mark(mark_ctx, "expand");
+    hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 16384), dim3(256), 0, st, (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
                        (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
                        (IRes *)W.mem[W_IRES]);
     PHX_CK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce, dim3(4096), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap,
+    if (mark) mark(mark_ctx, "items");
+    hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 4096), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap,
                        (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES], cnt + C_ITEMS, W.item_cap,
                        (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
                        a.chi2_max, (Kept *)W.mem[W_KEPT],
                        cnt + C_KEPT, W.kept_cap);
     PHX_CK(hipGetLastError());
-    if (mark) mark(mark_ctx, "refine");
+    if (mark) mark(mark_ctx, "reduce");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;
-    hipLaunchKernelGGL(k_finish_fast, dim3(4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap,
+    hipLaunchKernelGGL(k_finish_fast, dim3(W.finish_blocks > 0 ? W.finish_blocks : 4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap,
                        trk, (const double *)T.mem[T_CTRK], a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min,
                        (PhxRow *)W.mem[W_ROWS], cnt + C_ROWS, W.row_cap);
     PHX_CK(hipGetLastError());
@@ -823,7 +872,8 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     uint64_t key = 1469598103934665603ULL;
     auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };
     mix((uint64_t)a.ns); mix((uint64_t)a.nc); mix((uint64_t)a.npts); mix((uint64_t)a.n_samp);
-    mix((uint64_t)a.exact_tracks); mix((uint64_t)W.scan_blocks);
+    mix((uint64_t)a.exact_tracks); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
+    mix((uint64_t)W.reduce_blocks); mix((uint64_t)W.finish_blocks);
     double dv[2] = {a.chi2_max, a.p_min};
     uint64_t bits[2];
     memcpy(bits, dv, sizeof bits);
@@ -858,6 +908,7 @@ int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
     PHX_CK(hipStreamSynchronize(st));
     W.last_cand = (long)c[C_CAND];
     W.last_items = (long)c[C_ITEMS];
+    W.last_l2 = (long)c[C_L2];
     *grow = 0;
     if ((long)c[C_CAND] > W.cand_cap) { W.cand_cap = (long)c[C_CAND] + (long)c[C_CAND] / 4 + 4096; *grow = 1; }
     if ((long)c[C_ITEMS] > W.item_cap) { W.item_cap = (long)c[C_ITEMS] + (long)c[C_ITEMS] / 4 + 4096; *grow = 1; }

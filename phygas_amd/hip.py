@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         "phx_clear_stars": (_I, [_P]), "phx_screen": (_I, [_P, _I, _D, _I, _D]),
         "phx_rows": (_I, [_P, _P, _L, C.POINTER(_L)]), "phx_kept": (_I, [_P, C.POINTER(_L)]),
         "phx_candidates": (_I, [_P, C.POINTER(_L)]),
+        "phx_autotune": (_I, [_P, _I, _D, _I, _D, _P, _P]),
+        "phx_pass_stats": (_I, [_P, _P, _I]),
         "phx_stage_times": (_I, [_P, _P, _P, _I]), "phx_set_timing": (None, [_P, _I]),
         "phx_cluster_tracks": (_I, [_P, _P, _P, _P, _P, _I]), "phx_star_tracks": (_I, [_P, _I, _I, _P, _I]),
         "phx_chi2_matrix": (_I, [_P, _P, _P, _P]), "phx_npts": (_I, [_P]),
@@ -186,6 +188,21 @@ class Engine:
         n = _L(0)
         _check(self._L.phx_candidates(self.h, C.byref(n)), "phx_candidates")
         return n.value
+
+    def pass_stats(self) -> dict:
+        """Work counts of the last pass (see phx_pass_stats)."""
+        st = np.zeros(6, np.int64)
+        _check(self._L.phx_pass_stats(self.h, st.ctypes.data, 6), "phx_pass_stats")
+        return dict(zip(("l1_tests", "l2_tests", "candidates", "items", "kept", "rows"), (int(x) for x in st)))
+
+    def autotune(self, form: int = FAST, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> dict:
+        """Pick the fast pass's launch grids by timing its stages on the loaded batch; returns the choices."""
+        grids = np.zeros(4, np.int32)
+        ms = np.zeros(4)
+        _check(self._L.phx_autotune(self.h, form, chi2_max, n_samp, p_min, grids.ctypes.data, ms.ctypes.data),
+               "phx_autotune")
+        return {f"{k}_blocks": int(g) for k, g in zip(("scan", "items", "reduce", "finish"), grids)} | \
+            {f"{k}_ms": float(t) for k, t in zip(("scan", "items", "reduce", "finish"), ms)}
 
     def sync(self):
         _check(self._L.phx_sync(self.h), "phx_sync")

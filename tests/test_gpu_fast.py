@@ -151,6 +151,30 @@ def test_fast_vs_plain_on_work_unit_sample(unit):
     assert c2.max() < 1e-4
 
 
+def test_autotune_keeps_rows_and_stats_consistent(unit):
+    """Launch grids chosen by the autotuner change no row; the pass's work counts are consistent."""
+    cl, st, ctl = unit
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        before = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+        stats = e.pass_stats()
+        tune = e.autotune(hip.FAST, 25.0, 64, 0.0)
+        after = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+        stats2 = e.pass_stats()
+    assert before.tobytes() == after.tobytes()
+    assert stats == stats2
+    cn = np.ceil(cl["tmax_myr"] / 0.5).astype(np.int64) + 1
+    assert stats["l1_tests"] == (len(st) + len(ctl)) * int(np.sum((cn + 7) // 8))
+    assert stats["candidates"] <= stats["l2_tests"] <= stats["l1_tests"]
+    assert stats["candidates"] <= stats["items"] <= stats["l2_tests"]
+    assert stats["rows"] == len(before) and stats["kept"] >= stats["rows"]
+    for k in ("scan", "items", "reduce", "finish"):
+        assert tune[f"{k}_blocks"] > 0 and tune[f"{k}_ms"] > 0
+        _report(f"autotune_{k}_blocks", tune[f"{k}_blocks"])
+
+
 @pytest.fixture(scope="module")
 def pulsar_unit(workload):
     """1 synthetic pulsar x 300 clusters x 200 clones, dt 0.05 Myr (clones drawn on the host, seed per pair)."""

@@ -37,7 +37,7 @@ PLAIN_POINT_FLOP = 126.0   # k_chi2_plain, one interior pair-point (56 at a trac
 PLAIN_LAST_FLOP = 56.0
 SCAN_L1_FLOP = 76.0        # cull.h l1_pass, one (pair, tile) test, FP32
 SCAN_L2_FLOP = 177.0       # cull.h l2_pass, one (pair, tile) certificate, FP64
-ITEM_BYTES = 2 * 75 * 8    # k_items, one (pair, tile) item: the star's and the cluster's tiled records
+POINT_FLOP = 134.0         # k_items, one pair-point: both Mahalanobis forms and the segment parameter, FP64
 
 
 def plain_scan_flops(cn: np.ndarray, n_stars: int) -> float:
@@ -77,6 +77,27 @@ def cpu_baseline(clusters, stars, threads: int) -> dict | None:
                       f"and not timed"}
 
 
+def copy_probe(gib: float) -> int:
+    """One device-to-device copy of gib GiB through the HIP runtime (its blit kernel reads and writes exactly
+    that many bytes): the calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hiprt = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hiprt.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            print("hipMalloc failed", file=sys.stderr)
+            return 1
+    hiprt.hipMemset(a, 1, C.c_size_t(n))
+    hiprt.hipDeviceSynchronize()
+    rc = hiprt.hipMemcpy(b, a, C.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hiprt.hipDeviceSynchronize()
+    hiprt.hipFree(a)
+    hiprt.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n, "rc": rc}), flush=True)
+    return 0 if rc == 0 else 1
+
+
 def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float, p_min: float):
     for _ in range(warmup):
         e.screen_async(form, chi2_max, 64, p_min)
@@ -109,12 +130,18 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-rate", type=float, default=15.0)
-    ap.add_argument("--no-tune", action="store_true", help="default launch grids instead of the autotuner's")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner before timing (the library's default "
+                    "grids are its choices on this unit)")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB GiB device to device (the HBM counter calibration of the profiles) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the screening shards by star with no "
               f"collective, see the sharding paragraph of the design notes", file=sys.stderr)
         return 2
+
+    if args.copy_probe > 0:
+        return copy_probe(args.copy_probe)
 
     clusters = wl.make_clusters(4087, 0)
     stars = wl.make_stars(args.stars, 2)
@@ -133,7 +160,7 @@ def main() -> int:
 
     form = hip.FAST if args.form == "fast" else hip.PLAIN
     tune = None
-    if form == hip.FAST and not args.no_tune:
+    if form == hip.FAST and args.tune:
         tune = e.autotune(hip.FAST, chi2_max, 64, p_min)   # launch grids, chosen before the timed steps
     ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
     rows = e.rows()
@@ -168,11 +195,11 @@ def main() -> int:
         name = max(("scan", "items"), key=lambda k: stages.get(k, 0.0))
         k_ms = stages.get(name, float("nan"))
         if name == "items":
-            work = stats["items"] * ITEM_BYTES
-            achieved = work / (k_ms / 1000.0) / 1e9
-            roof = {"bound": "hbm", "kernel": "k_items", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
-                    "frac": achieved / PEAK_HBM, "traffic": pmc_traffic("k_items"), "work_per_launch": work,
-                    "avg_launch_ms": k_ms}
+            work = stats["items"] * 8 * POINT_FLOP   # eight points per item (fewer at a track's end: an upper bound)
+            achieved = work / (k_ms / 1000.0) / 1e12
+            roof = {"bound": "valu", "kernel": "k_items", "achieved": achieved, "peak": PEAK_VALU,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_items"),
+                    "work_per_launch": work, "avg_launch_ms": k_ms}
         else:
             work = stats["l1_tests"] * SCAN_L1_FLOP + stats["l2_tests"] * SCAN_L2_FLOP
             achieved = work / (k_ms / 1000.0) / 1e12

@@ -17,8 +17,9 @@
 //                    Mahalanobis forms per point, strict-< first-index minimum)
 //      k_reduce      one lane per candidate: minimum over its items; pairs with chi2 < chi2_max are appended
 //                    to the kept list
-//   6. k_finish_fast eight lanes per kept pair: the 512 Box-Muller draws of the pair's seed, screened draw
-//                    by draw (below), exact counts; rows with p_r50 >= p_min appended
+//   6. k_finish_fast one wavefront per 8 kept pairs: the 512 Box-Muller draws of each pair's seed, screened
+//                    draw by draw (below) with the rare survivors evaluated through a wave-local queue;
+//                    exact counts; rows with p_r50 >= p_min appended
 //
 // Exactness of the cut: a tile dropped by the scan holds no point with chi2 < chi2_max (cull.h), and the
 // refine's point test drops a point only when chi2 >= d^2 / lambda_max(M_k) > chi2_max, with d the distance
@@ -417,7 +418,7 @@ __device__ __forceinline__ void tiled_point(const double *__restrict__ ts, const
 }
 
 struct Item { int cand, star, cluster, j; };
-struct IRes { double q; int k, pad; };
+struct IRes { double q, sv; int k, pad; };
 
 // Items of the candidates, one per tile left by the scan, contiguous per candidate in increasing tile
 // order: istart[c] = first item of candidate c.  Grid-stride over blocks of 256 candidates; each block takes
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
     const int gpb = blockDim.x >> 3;   // items per workgroup pass
     for (long b0 = c0 + (long)(blockIdx.x >> 3) * gpb; b0 < c1; b0 += (long)nbx * gpb) {
         const long id = b0 + (threadIdx.x >> 3);
-        double q = __builtin_inf();
+        double q = __builtin_inf(), sv = 0.0;
         int k = 0;
         if (id < c1) {
             const Item it = items[id];
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
             k = it.j * TILE + p;
             if (k < np) {
                 V3 m0, m1;
-                double M[6], sv;
+                double M[6];
                 tiled_point(stile + ((size_t)it.star * ntl + it.j) * TREC, ctile + ((size_t)it.cluster * ntl + it.j) * TREC,
                             p, m0, m1, M);
                 const double v = point_chi2(m0, m1, k + 1 < np, M, R2s[it.cluster], &sv);
@@ -498,24 +499,21 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
         }
 #pragma unroll
         for (int d = 1; d < 8; d <<= 1) {
-            const double qo = __shfl_xor(q, d, 64);
+            const double qo = __shfl_xor(q, d, 64), so = __shfl_xor(sv, d, 64);
             const int ko = __shfl_xor(k, d, 64);
-            if (qo < q || (qo == q && ko < k)) { q = qo; k = ko; }
+            if (qo < q || (qo == q && ko < k)) { q = qo; k = ko; sv = so; }
         }
-        if (p == 0 && id < c1) res[id] = IRes{q, k, 0};
+        if (p == 0 && id < c1) res[id] = IRes{q, sv, k, 0};
     }
 }
 
-// One lane per candidate: minimum over its items in tile order; kept pairs get the segment parameter of
-// their minimum re-evaluated and are appended to the kept list (one atomic per wavefront).
+// One lane per candidate: minimum over its items in tile order (the items carry the segment parameter of
+// their minimum); kept pairs are appended to the kept list (one atomic per wavefront).
 __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
                                                 long cand_cap, const long *__restrict__ istart,
                                                 const IRes *__restrict__ res, const unsigned long long *__restrict__ nitems,
-                                                long item_cap, const double *__restrict__ stile,
-                                                const double *__restrict__ ctile, const int *__restrict__ cn,
-                                                const double *__restrict__ R2s, int npts, int ntl, double chi2_max,
-                                                Kept *__restrict__ kept, unsigned long long *__restrict__ nkept,
-                                                long kept_cap) {
+                                                long item_cap, double chi2_max, Kept *__restrict__ kept,
+                                                unsigned long long *__restrict__ nkept, long kept_cap) {
     const long n = min((long)*ncand, cand_cap);
     if ((long)*nitems > item_cap) return;   // the pass is run again with a larger item list
     const int lane = threadIdx.x & 63;
@@ -528,20 +526,18 @@ __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, c
             const Cand cd = cand[id];
             const long i0 = istart[id];
             const int ni = __popcll(cd.mask);
-            double best = __builtin_inf();
+            double best = __builtin_inf(), sb = 0.0;
             int kb = 0;
-            for (int i = 0; i < ni; i++) {
-                const IRes r = res[i0 + i];
-                if (r.q < best) { best = r.q; kb = r.k; }
+            for (int i = 0; i < ni; i += 4) {   // four results in flight
+                IRes r[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) r[u] = i + u < ni ? res[i0 + i + u] : IRes{__builtin_inf(), 0.0, 0, 0};
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (r[u].q < best) { best = r[u].q; kb = r[u].k; sb = r[u].sv; }
             }
             if (best < chi2_max) {
-                const int c = cd.cluster, np = min(npts, cn[c]), j = kb / TILE;
-                V3 m0, m1;
-                double M[6], sv;
-                tiled_point(stile + ((size_t)cd.star * ntl + j) * TREC, ctile + ((size_t)c * ntl + j) * TREC, kb % TILE, m0,
-                            m1, M);
-                (void)point_chi2(m0, m1, kb + 1 < np, M, R2s[c], &sv);
-                kp = Kept{cd.star, c, kb, 0, sv, best};
+                kp = Kept{cd.star, cd.cluster, kb, 0, sb, best};
                 hit = true;
             }
         }
@@ -638,7 +634,74 @@ __device__ int count_inside8(uint64_t seed, const double l[6], V3 m, double r, d
     return cnt;
 }
 
-// Eight lanes per kept pair (a group): every lane derives the pair's frame, the group counts the draws.
+// Per-pair frame of the finish, shared by the lanes of a wavefront through LDS.
+struct FinPar {
+    uint64_t seed, u1max;
+    double mx, my, mz, l[6], r, ra, marg;
+    float flo, fhi;
+    int all, none;
+};
+
+// Screens for one radius pair (r, ra <= r): the z0 interval of |m.x + l00 z0| < r, the integer u1 bound and
+// the margin of the whole-draw estimate.  none = 1: no draw can count.
+__device__ void fin_setup(FinPar &f, uint64_t seed, const double L[6], V3 m, double r, double ra) {
+    f.seed = seed;
+    f.mx = m.x; f.my = m.y; f.mz = m.z;
+#pragma unroll
+    for (int i = 0; i < 6; i++) f.l[i] = L[i];
+    f.r = r; f.ra = ra;
+    const double l00 = L[0];
+    double lo, hi;
+    bool all = false;
+    if (l00 > 0) {
+        lo = (-r - m.x) / l00;
+        hi = (r - m.x) / l00;
+    } else {
+        all = fabs(m.x) < r;
+        lo = -1e300;
+        hi = 1e300;
+    }
+    // |z0| <= 9.42 for every representable u1; with l00 = 0, x0 = m.x for every draw
+    f.none = (l00 > 0 && (lo >= 9.5 || hi <= -9.5)) || (l00 <= 0 && !all);
+    f.all = all;
+    const double a = all ? 0.0 : (lo > 0 ? lo : (hi < 0 ? -hi : 0.0));
+    const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u1 above this: |z0| < a
+    f.u1max = ucut >= 1.0 ? ~0ull : (uint64_t)(ucut * 9007199254740992.0) + 1;
+    f.flo = (float)lo - 1e-3f;
+    f.fhi = (float)hi + 1e-3f;
+    f.marg = 1e-3 * (fabs(L[0]) + fabs(L[1]) + fabs(L[2]) + fabs(L[3]) + fabs(L[4]) + fabs(L[5])) + 1e-12 * r;
+}
+
+// Draw i of a pair whose z0 estimate fell in the x interval (w1, w2: its u1 and u2 bits): bit 0 when it
+// counts for r, bit 1 for ra.
+__device__ int fin_draw(const FinPar &f, int i, uint64_t w1, uint64_t w2) {
+    const uint64_t b = f.seed + (uint64_t)(6 * i) * GOLDEN_GAMMA;
+    const double u1 = (double)w1 * (1.0 / 9007199254740992.0), u2 = (double)w2 * (1.0 / 9007199254740992.0);
+    const double u3 = to_unit(mix64(b + 3 * GOLDEN_GAMMA)), u4 = to_unit(mix64(b + 4 * GOLDEN_GAMMA));
+    const double u5 = to_unit(mix64(b + 5 * GOLDEN_GAMMA)), u6 = to_unit(mix64(b + 6 * GOLDEN_GAMMA));
+    {   // FP32 estimate of the whole draw, decided unless within the margin of a radius
+        const double e0 = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, u2);
+        const double e1 = z0_estimate(u3 <= 0.0 ? 1.0 / 18446744073709551616.0 : u3, u4);
+        const double e2 = z0_estimate(u5 <= 0.0 ? 1.0 / 18446744073709551616.0 : u5, u6);
+        const double xe = f.mx + f.l[0] * e0, ye = f.my + f.l[1] * e0 + f.l[3] * e1,
+                     ze = f.mz + f.l[2] * e0 + f.l[4] * e1 + f.l[5] * e2;
+        const double re = sqrt(xe * xe + ye * ye + ze * ze);
+        const bool sure_r = re + f.marg < f.r || re - f.marg >= f.r, sure_a = re + f.marg < f.ra || re - f.marg >= f.ra;
+        if (sure_r && sure_a) return (re + f.marg < f.r ? 1 : 0) | (re + f.marg < f.ra ? 2 : 0);
+    }
+    const double z0 = box_muller(u1, u2), z1 = box_muller(u3, u4), z2 = box_muller(u5, u6);
+    const double x = f.mx + f.l[0] * z0, y = f.my + f.l[1] * z0 + f.l[3] * z1,
+                 z = f.mz + f.l[2] * z0 + f.l[4] * z1 + f.l[5] * z2;
+    const double rr = sqrt(x * x + y * y + z * z);
+    return (rr < f.r ? 1 : 0) | (rr < f.ra ? 2 : 0);
+}
+
+// One wavefront per 8 kept pairs.  Lane p of group g screens draws p, p+8, ... of pair g (512 per pair):
+// u1 against the integer bound, then the FP32 z0 estimate against the x interval; the few draws left go to
+// a wave-local queue in LDS, evaluated 64 entries at a time (one per lane), so that the whole-draw estimate
+// and the exact path run on full wavefronts.  Pairs
+// of the p_min mode whose r50 count reaches n50_min have their r_tot count taken by the group (rare).
+constexpr int FQ = 128;   // queue ring per wavefront
 __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ kept,
                                                      const unsigned long long *__restrict__ nkept, long kept_cap,
                                                      const double *__restrict__ strk, const double *__restrict__ ctrk,
@@ -649,11 +712,21 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
                                                      const int *__restrict__ sgroup, int npts, int n50_min,
                                                      PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
                                                      long row_cap) {
+    __shared__ FinPar par_s[4][8];
+    __shared__ uint64_t qw_s[4][FQ], qv_s[4][FQ];
+    __shared__ unsigned short qi_s[4][FQ];
+    __shared__ int cnt_s[4][8], alt_s[4][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
+    FinPar *par = par_s[w];
+    uint64_t *qw = qw_s[w], *qv = qv_s[w];
+    unsigned short *qi = qi_s[w];
     const long nk = min((long)*nkept, kept_cap);
-    const int p = threadIdx.x & 7;
-    const long g0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3, ng = ((long)gridDim.x * blockDim.x) >> 3;
-    for (long h = g0; h < nk; h += ng) {   // uniform within a group: its lanes shuffle with each other only
-        const Kept kp = kept[h];
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (long h0 = wave * 8; h0 < nk; h0 += nwave * 8) {   // wave-uniform
+        const long h = h0 + g;
+        const bool live = h < nk;
+        const Kept kp = kept[live ? h : h0];
         const int s = kp.star, c = kp.cluster, kb = kp.kb;
         const int n = min(npts, cn[c]);
         const int k1 = kb + 1 < n ? kb + 1 : kb;
@@ -673,28 +746,96 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         L[5] = sqrt(fmax(C[5] - L[2] * L[2] - L[4] * L[4], 0.0));
         const uint64_t seed = fnv_feed(sprefix[s], cl[c].name, 40);
         const double r50 = r50s[c], rt = rts[c];
-        int in50, intot;
-        if (n50_min > 0) {
-            in50 = count_inside8(seed, L, m, r50, r50, nullptr, p);
-            if (in50 < n50_min) continue;
-            intot = count_inside8(seed, L, m, rt, rt, nullptr, p);
-        } else {   // every kept pair is a row: count both, screening with the larger radius
-            const int n_big = count_inside8(seed, L, m, fmax(rt, r50), rt < r50 ? rt : r50, &in50, p);
-            if (rt < r50) { intot = in50; in50 = n_big; } else intot = n_big;
+        const double rbig = n50_min > 0 ? r50 : fmax(rt, r50), rsmall = n50_min > 0 ? r50 : (rt < r50 ? rt : r50);
+        if (p == 0) {
+            fin_setup(par[g], seed, L, m, rbig, rsmall);
+            if (!live) par[g].none = 1;
+            cnt_s[w][g] = 0;
+            alt_s[w][g] = 0;
         }
-        if (p != 0) continue;
-        PhxRow o;
-        o.star = s; o.cluster = c; o.kb = kb; o.flags = sgroup[s];
-        o.chi2 = kp.chi2; o.sb = kp.sb;
-        o.t = (kb + kp.sb) * STEP_MYR;
-        o.d_nom_pc = norm(m) * 1000.0;
-        o.sig_pc = sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0;
-        o.p_r50 = (double)in50 / 512;
-        o.p_rtot = (double)intot / 512;
-        const V3 dv = v3(ps[kb * 12 + 3] - pc[kb * 12 + 3], ps[kb * 12 + 4] - pc[kb * 12 + 4], ps[kb * 12 + 5] - pc[kb * 12 + 5]);
-        o.vrel = norm(dv) * KMS_PER_KPCMYR;
-        const unsigned long long ro = atomicAdd(nrows, 1ull);
-        if ((long)ro < row_cap) rows[ro] = o;
+        wave_sync();
+        const bool none = par[g].none, all = par[g].all;
+        const uint64_t u1max = par[g].u1max;
+        const float flo = par[g].flo, fhi = par[g].fhi;
+        int head = 0, tail = 0;   // wave-uniform
+        for (int t = 0; t < 64; t++) {
+            const int i = p + 8 * t;
+            uint64_t w1 = 0, w2 = 0;
+            bool pass = false;
+            if (!none) {
+                w1 = mix64(seed + (uint64_t)(6 * i + 1) * GOLDEN_GAMMA) >> 11;
+                if (w1 <= u1max) {   // a draw whose z0 lies outside the x interval cannot count
+                    w2 = mix64(seed + (uint64_t)(6 * i + 2) * GOLDEN_GAMMA) >> 11;
+                    const double u1 = (double)w1 * (1.0 / 9007199254740992.0);
+                    const float ze0 = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1,
+                                                  (double)w2 * (1.0 / 9007199254740992.0));
+                    pass = all || (ze0 >= flo && ze0 <= fhi);
+                }
+            }
+            const unsigned long long bal = __ballot(pass);
+            if (pass) {
+                const int pos = (tail + __popcll(bal & below)) & (FQ - 1);
+                qw[pos] = w1;
+                qv[pos] = w2;
+                qi[pos] = (unsigned short)(g << 9 | i);
+            }
+            tail += __popcll(bal);
+            if (tail - head >= 64 || (t == 63 && tail > head)) {   // wave-uniform
+                wave_sync();
+                const int e = head + lane;
+                if (e < tail) {
+                    const int pos = e & (FQ - 1);
+                    const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
+                    const int r = fin_draw(par[gg], ii, qw[pos], qv[pos]);
+                    if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
+                    if (r & 2) atomicAdd(&alt_s[w][gg], 1);
+                }
+                head = min(head + 64, tail);
+                wave_sync();
+            }
+        }
+        // the last flush may leave entries when more than 64 were pending
+        while (tail > head) {   // wave-uniform
+            wave_sync();
+            const int e = head + lane;
+            if (e < tail) {
+                const int pos = e & (FQ - 1);
+                const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
+                const int r = fin_draw(par[gg], ii, qw[pos], qv[pos]);
+                if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
+                if (r & 2) atomicAdd(&alt_s[w][gg], 1);
+            }
+            head = min(head + 64, tail);
+            wave_sync();
+        }
+        wave_sync();
+        int in50, intot;
+        const int cbig = cnt_s[w][g], csmall = alt_s[w][g];
+        bool row = live;
+        if (n50_min > 0) {
+            in50 = cbig;
+            row = row && in50 >= n50_min;
+            intot = 0;
+            if (row) intot = count_inside8(seed, L, m, rt, rt, nullptr, p);   // group-uniform, rare
+        } else {
+            if (rt < r50) { intot = csmall; in50 = cbig; } else { intot = cbig; in50 = csmall; }
+        }
+        if (row && p == 0) {
+            PhxRow o;
+            o.star = s; o.cluster = c; o.kb = kb; o.flags = sgroup[s];
+            o.chi2 = kp.chi2; o.sb = kp.sb;
+            o.t = (kb + kp.sb) * STEP_MYR;
+            o.d_nom_pc = norm(m) * 1000.0;
+            o.sig_pc = sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0;
+            o.p_r50 = (double)in50 / 512;
+            o.p_rtot = (double)intot / 512;
+            const V3 dv = v3(ps[kb * 12 + 3] - pc[kb * 12 + 3], ps[kb * 12 + 4] - pc[kb * 12 + 4],
+                             ps[kb * 12 + 5] - pc[kb * 12 + 5]);
+            o.vrel = norm(dv) * KMS_PER_KPCMYR;
+            const unsigned long long ro = atomicAdd(nrows, 1ull);
+            if ((long)ro < row_cap) rows[ro] = o;
+        }
+        wave_sync();   // par / counters are rewritten by the next batch
     }
 }
 
@@ -836,16 +977,15 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
                        cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "expand");
-    hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 16384), dim3(256), 0, st, (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
+    hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 32768), dim3(256), 0, st,
+                       (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
                        (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
                        (IRes *)W.mem[W_IRES]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "items");
-    hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 4096), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap,
-                       (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES], cnt + C_ITEMS, W.item_cap,
-                       (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
-                       a.chi2_max, (Kept *)W.mem[W_KEPT],
-                       cnt + C_KEPT, W.kept_cap);
+    hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 8192), dim3(256), 0, st, cand,
+                       cnt + C_CAND, W.cand_cap, (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES],
+                       cnt + C_ITEMS, W.item_cap, a.chi2_max, (Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "reduce");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;

@@ -170,6 +170,31 @@ int phx_server_pair_results(PhxServer *srv, long id, PhResult *res, int cap);
 int phx_server_info(PhxServer *srv, long id, double t[3], int *batch_reqs);
 int phx_server_release(PhxServer *srv, long id);
 
+/* ---------------------------------------------------------------- 4. work units
+ * The reference app's task (reference: src/main_phygas_boinc.cpp:336-471) over the batched engine: the `in` file
+ * (header "PHYGAS 1 <screen|pair> p1 p2 p3 p4", targets with optional clusters= lists) and the cluster file in,
+ * the app's rows (10 columns for screen, 15 for pair) and "# done <nt>" out, in target order.  Screen mode: p1
+ * chi2_max, p2 n_samp, p3 kernel radius (<= 0: 2), p4 p_min.  Pair mode: p1 clones, p2 dt (Myr); targets without
+ * a clusters= list take the device prefilter.  After each batch of targets a checkpoint line
+ * "phygas_checkpoint 2 <targets_done> <lines_done>" is written atomically; a resumed run keeps exactly that many
+ * complete lines of `out`, so no row is lost or duplicated. */
+typedef struct {
+    int form;             /* PHX_FORM_FAST (default), PHX_FORM_FAST64 or PHX_FORM_PLAIN */
+    int batch_targets;    /* targets per GPU batch (checkpoint interval); default 20,000 */
+    int max_batches;      /* > 0: stop (return 1) after this many batches, checkpoint written */
+    int device;
+    int draw_on_device;   /* pair mode: clones drawn on the GPU (PHX_DRAW_ON_DEVICE) instead of the host */
+} PhxWuOptions;
+void phx_wu_defaults(PhxWuOptions *opt);
+/* 0: done (out ends with "# done"); 1: stopped after max_batches; 4: the checkpoint exists but cannot be read
+ * (nothing touched: retry later); other: error (phx_last_error()).  *targets_done: targets in out. */
+int phx_wu_run(const char *in_path, const char *clusters_path, const char *out_path, const char *ckpt_path,
+               const PhxWuOptions *opt, long *targets_done);
+/* The resume step alone (host only): reads the checkpoint, trims `out` to the lines it counts (or truncates it
+ * for a fresh start) and returns where to continue.  No counterpart in the reference (its app resumes from a bare
+ * target count and keeps every row written after it). */
+int phx_wu_resume(const char *out_path, const char *ckpt_path, long nt, long *targets_done, long *lines_kept);
+
 /* Pair-mode prefilter on the device (reference: src/phygas.c:98-168 ph_track_* and ph_prefilter, as
  * src/phygas_cli.c:103-117 runs them): mask[j * nc + i] = 1 when target j may have reached cluster i. */
 int phx_prefilter(PhxCtx *ctx, const PhTarget *t, int nt, const PhCluster *c, int nc, unsigned char *mask);

@@ -63,6 +63,8 @@ def lib() -> C.CDLL:
         "phx_candidates": (_I, [_P, C.POINTER(_L)]),
         "phx_autotune": (_I, [_P, _I, _D, _I, _D, _P, _P]),
         "phx_pass_stats": (_I, [_P, _P, _I]),
+        "phx_wu_run": (_I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _P, C.POINTER(_L)]),
+        "phx_wu_resume": (_I, [C.c_char_p, C.c_char_p, _L, C.POINTER(_L), C.POINTER(_L)]),
         "phx_stage_times": (_I, [_P, _P, _P, _I]), "phx_set_timing": (None, [_P, _I]),
         "phx_cluster_tracks": (_I, [_P, _P, _P, _P, _P, _I]), "phx_star_tracks": (_I, [_P, _I, _I, _P, _I]),
         "phx_chi2_matrix": (_I, [_P, _P, _P, _P]), "phx_npts": (_I, [_P]),
@@ -110,6 +112,36 @@ def pair_draw(t, c, n_clones: int, dt_myr: float, seed_: int):
     dt = np.zeros(1)
     m = lib().phx_pair_draw(_p(t), _p(c), n_clones, dt_myr, seed_, _p(ps), _p(steps), _p(dt))
     return ps[:m], int(steps[0]), float(dt[0])
+
+
+class WuOptions(C.Structure):
+    """PhxWuOptions of include/phygas_hip.h."""
+    _fields_ = [("form", C.c_int), ("batch_targets", C.c_int), ("max_batches", C.c_int), ("device", C.c_int),
+                ("draw_on_device", C.c_int)]
+
+
+def wu_run(in_path: str, clusters_path: str, out_path: str, ckpt_path: str, form: int = FAST,
+           batch_targets: int = 20000, max_batches: int = 0, device: int = 0, draw_on_device: bool = False) -> tuple[int, int]:
+    """Run a work unit (phx_wu_run): returns (status, targets done); status 0 = done, 1 = stopped after
+    max_batches, 4 = the checkpoint exists but cannot be read.  Other codes raise."""
+    L = lib()
+    o = WuOptions(form, batch_targets, max_batches, device, int(draw_on_device))
+    done = _L(0)
+    rc = L.phx_wu_run(in_path.encode(), clusters_path.encode(), out_path.encode(), ckpt_path.encode(),
+                      C.addressof(o), C.byref(done))
+    if rc not in (0, 1, 4):
+        _check(rc, "phx_wu_run")
+    return rc, done.value
+
+
+def wu_resume(out_path: str, ckpt_path: str, nt: int) -> tuple[int, int, int]:
+    """The resume step alone (phx_wu_resume): (status, targets done, lines kept)."""
+    L = lib()
+    t, n = _L(0), _L(0)
+    rc = L.phx_wu_resume(out_path.encode(), ckpt_path.encode(), nt, C.byref(t), C.byref(n))
+    if rc not in (0, 4):
+        _check(rc, "phx_wu_resume")
+    return rc, t.value, n.value
 
 
 class Engine:

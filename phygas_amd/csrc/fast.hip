@@ -639,44 +639,67 @@ struct FinPar {
     uint64_t seed, u1max;
     double mx, my, mz, l[6], r, ra, marg;
     float flo, fhi;
-    int all, none;
+    int all, none, axis;
 };
 
-// Screens for one radius pair (r, ra <= r): the z0 interval of |m.x + l00 z0| < r, the integer u1 bound and
-// the margin of the whole-draw estimate.  none = 1: no draw can count.
+// Screens for one radius pair (r, ra <= r).  A draw counts only when z = L^-1 (x - m) lies in the ellipsoid
+// E = c + L^-1 B(0, r), c = -L^-1 m, so its normal coordinate z_k lies in the projection of E on axis k,
+// I_k = c_k +- r |row k of L^-1|.  z_k = rho_k cos(.) with rho_k = sqrt(-2 ln u) from the draw's (2k+1)-th
+// uniform u: a draw whose u exceeds exp(-dist(0, I_k)^2 / 2) cannot count.  The axis with the farthest
+// interval screens (one hash per draw for most draws), then the FP32 estimate of z_k against I_k.  With a
+// singular L only the x axis is used (I_0 from |m.x + l00 z0| < r).  none = 1: no draw can count.
 __device__ void fin_setup(FinPar &f, uint64_t seed, const double L[6], V3 m, double r, double ra) {
     f.seed = seed;
     f.mx = m.x; f.my = m.y; f.mz = m.z;
 #pragma unroll
     for (int i = 0; i < 6; i++) f.l[i] = L[i];
     f.r = r; f.ra = ra;
-    const double l00 = L[0];
+    f.marg = 1e-3 * (fabs(L[0]) + fabs(L[1]) + fabs(L[2]) + fabs(L[3]) + fabs(L[4]) + fabs(L[5])) + 1e-12 * r;
     double lo, hi;
     bool all = false;
-    if (l00 > 0) {
-        lo = (-r - m.x) / l00;
-        hi = (r - m.x) / l00;
+    int axis = 0;
+    if (L[0] > 0 && L[3] > 0 && L[5] > 0) {
+        // L^-1 (lower triangular) and the three projections
+        const double i00 = 1.0 / L[0], i11 = 1.0 / L[3], i22 = 1.0 / L[5];
+        const double i10 = -i11 * L[1] * i00, i21 = -i22 * L[4] * i11, i20 = -(i21 * L[1] + i22 * L[2]) * i00;
+        const double c[3] = {-i00 * m.x, -(i10 * m.x + i11 * m.y), -(i20 * m.x + i21 * m.y + i22 * m.z)};
+        const double h[3] = {r * i00, r * sqrt(i10 * i10 + i11 * i11), r * sqrt(i20 * i20 + i21 * i21 + i22 * i22)};
+        // row magnitudes (including the terms i20 is summed from) times the size of m and of the computed
+        // positions bound the rounding of c_k, h_k and of x = m + L z far below this padding
+        const double mabs = fabs(m.x) + fabs(m.y) + fabs(m.z) + r +
+                            10.0 * (fabs(L[0]) + fabs(L[1]) + fabs(L[2]) + fabs(L[3]) + fabs(L[4]) + fabs(L[5]));
+        const double rowa[3] = {i00, fabs(i10) + i11,
+                                (fabs(i21 * L[1]) + fabs(i22 * L[2])) * i00 + fabs(i21) + i22};
+        double best = -1.0;
+        for (int k = 0; k < 3; k++) {
+            const double pad = 1e-9 * (fabs(c[k]) + h[k] + rowa[k] * mabs);
+            const double lk = c[k] - h[k] - pad, hk = c[k] + h[k] + pad;
+            const double dk = lk > 0 ? lk : (hk < 0 ? -hk : 0.0);
+            if (dk > best) { best = dk; axis = k; lo = lk; hi = hk; }
+        }
+    } else if (L[0] > 0) {
+        lo = (-r - m.x) / L[0];
+        hi = (r - m.x) / L[0];
     } else {
         all = fabs(m.x) < r;
         lo = -1e300;
         hi = 1e300;
     }
-    // |z0| <= 9.42 for every representable u1; with l00 = 0, x0 = m.x for every draw
-    f.none = (l00 > 0 && (lo >= 9.5 || hi <= -9.5)) || (l00 <= 0 && !all);
+    // |z_k| <= 9.42 for every representable u; with l00 = 0, x0 = m.x for every draw
+    f.none = (!all && (lo >= 9.5 || hi <= -9.5)) || (L[0] <= 0 && !all);
     f.all = all;
+    f.axis = axis;
     const double a = all ? 0.0 : (lo > 0 ? lo : (hi < 0 ? -hi : 0.0));
-    const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u1 above this: |z0| < a
+    const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u above this: |z_k| < a
     f.u1max = ucut >= 1.0 ? ~0ull : (uint64_t)(ucut * 9007199254740992.0) + 1;
     f.flo = (float)lo - 1e-3f;
     f.fhi = (float)hi + 1e-3f;
-    f.marg = 1e-3 * (fabs(L[0]) + fabs(L[1]) + fabs(L[2]) + fabs(L[3]) + fabs(L[4]) + fabs(L[5])) + 1e-12 * r;
 }
 
-// Draw i of a pair whose z0 estimate fell in the x interval (w1, w2: its u1 and u2 bits): bit 0 when it
-// counts for r, bit 1 for ra.
-__device__ int fin_draw(const FinPar &f, int i, uint64_t w1, uint64_t w2) {
+// Draw i of a pair that passed the axis screen: bit 0 when it counts for r, bit 1 for ra.
+__device__ int fin_draw(const FinPar &f, int i) {
     const uint64_t b = f.seed + (uint64_t)(6 * i) * GOLDEN_GAMMA;
-    const double u1 = (double)w1 * (1.0 / 9007199254740992.0), u2 = (double)w2 * (1.0 / 9007199254740992.0);
+    const double u1 = to_unit(mix64(b + GOLDEN_GAMMA)), u2 = to_unit(mix64(b + 2 * GOLDEN_GAMMA));
     const double u3 = to_unit(mix64(b + 3 * GOLDEN_GAMMA)), u4 = to_unit(mix64(b + 4 * GOLDEN_GAMMA));
     const double u5 = to_unit(mix64(b + 5 * GOLDEN_GAMMA)), u6 = to_unit(mix64(b + 6 * GOLDEN_GAMMA));
     {   // FP32 estimate of the whole draw, decided unless within the margin of a radius
@@ -713,12 +736,10 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
                                                      PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
                                                      long row_cap) {
     __shared__ FinPar par_s[4][8];
-    __shared__ uint64_t qw_s[4][FQ], qv_s[4][FQ];
     __shared__ unsigned short qi_s[4][FQ];
     __shared__ int cnt_s[4][8], alt_s[4][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
     FinPar *par = par_s[w];
-    uint64_t *qw = qw_s[w], *qv = qv_s[w];
     unsigned short *qi = qi_s[w];
     const long nk = min((long)*nkept, kept_cap);
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
@@ -757,15 +778,18 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         const bool none = par[g].none, all = par[g].all;
         const uint64_t u1max = par[g].u1max;
         const float flo = par[g].flo, fhi = par[g].fhi;
+        const uint64_t ua = (uint64_t)(2 * par[g].axis + 1) * GOLDEN_GAMMA, ub = ua + GOLDEN_GAMMA;
         int head = 0, tail = 0;   // wave-uniform
-        for (int t = 0; t < 64; t++) {
+        // splitmix64 state before draw i's first uniform: seed + 6 i gamma, advanced by 48 gamma per step
+        uint64_t st = seed + (uint64_t)(6 * p) * GOLDEN_GAMMA;
+        for (int t = 0; t < 64; t++, st += 48 * GOLDEN_GAMMA) {
             const int i = p + 8 * t;
             uint64_t w1 = 0, w2 = 0;
             bool pass = false;
             if (!none) {
-                w1 = mix64(seed + (uint64_t)(6 * i + 1) * GOLDEN_GAMMA) >> 11;
-                if (w1 <= u1max) {   // a draw whose z0 lies outside the x interval cannot count
-                    w2 = mix64(seed + (uint64_t)(6 * i + 2) * GOLDEN_GAMMA) >> 11;
+                w1 = mix64(st + ua) >> 11;
+                if (w1 <= u1max) {   // a draw whose z_k lies outside I_k cannot count
+                    w2 = mix64(st + ub) >> 11;
                     const double u1 = (double)w1 * (1.0 / 9007199254740992.0);
                     const float ze0 = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1,
                                                   (double)w2 * (1.0 / 9007199254740992.0));
@@ -775,8 +799,6 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
             const unsigned long long bal = __ballot(pass);
             if (pass) {
                 const int pos = (tail + __popcll(bal & below)) & (FQ - 1);
-                qw[pos] = w1;
-                qv[pos] = w2;
                 qi[pos] = (unsigned short)(g << 9 | i);
             }
             tail += __popcll(bal);
@@ -786,7 +808,7 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
                 if (e < tail) {
                     const int pos = e & (FQ - 1);
                     const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
-                    const int r = fin_draw(par[gg], ii, qw[pos], qv[pos]);
+                    const int r = fin_draw(par[gg], ii);
                     if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
                     if (r & 2) atomicAdd(&alt_s[w][gg], 1);
                 }
@@ -801,7 +823,7 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
             if (e < tail) {
                 const int pos = e & (FQ - 1);
                 const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
-                const int r = fin_draw(par[gg], ii, qw[pos], qv[pos]);
+                const int r = fin_draw(par[gg], ii);
                 if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
                 if (r & 2) atomicAdd(&alt_s[w][gg], 1);
             }

@@ -736,11 +736,13 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
                                                      PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
                                                      long row_cap) {
     __shared__ FinPar par_s[4][8];
-    __shared__ unsigned short qi_s[4][FQ];
+    __shared__ uint64_t qw_s[4][FQ];
+    __shared__ unsigned short qi_s[4][FQ], qx_s[4][FQ];
     __shared__ int cnt_s[4][8], alt_s[4][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
     FinPar *par = par_s[w];
-    unsigned short *qi = qi_s[w];
+    uint64_t *qw = qw_s[w];
+    unsigned short *qi = qi_s[w], *qx = qx_s[w];
     const long nk = min((long)*nkept, kept_cap);
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -775,61 +777,63 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
             alt_s[w][g] = 0;
         }
         wave_sync();
-        const bool none = par[g].none, all = par[g].all;
+        const bool none = par[g].none;
         const uint64_t u1max = par[g].u1max;
-        const float flo = par[g].flo, fhi = par[g].fhi;
-        const uint64_t ua = (uint64_t)(2 * par[g].axis + 1) * GOLDEN_GAMMA, ub = ua + GOLDEN_GAMMA;
-        int head = 0, tail = 0;   // wave-uniform
-        // splitmix64 state before draw i's first uniform: seed + 6 i gamma, advanced by 48 gamma per step
-        uint64_t st = seed + (uint64_t)(6 * p) * GOLDEN_GAMMA;
-        for (int t = 0; t < 64; t++, st += 48 * GOLDEN_GAMMA) {
-            const int i = p + 8 * t;
-            uint64_t w1 = 0, w2 = 0;
-            bool pass = false;
-            if (!none) {
-                w1 = mix64(st + ua) >> 11;
-                if (w1 <= u1max) {   // a draw whose z_k lies outside I_k cannot count
-                    w2 = mix64(st + ub) >> 11;
-                    const double u1 = (double)w1 * (1.0 / 9007199254740992.0);
-                    const float ze0 = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1,
-                                                  (double)w2 * (1.0 / 9007199254740992.0));
-                    pass = all || (ze0 >= flo && ze0 <= fhi);
-                }
-            }
-            const unsigned long long bal = __ballot(pass);
-            if (pass) {
-                const int pos = (tail + __popcll(bal & below)) & (FQ - 1);
-                qi[pos] = (unsigned short)(g << 9 | i);
-            }
-            tail += __popcll(bal);
-            if (tail - head >= 64 || (t == 63 && tail > head)) {   // wave-uniform
-                wave_sync();
-                const int e = head + lane;
-                if (e < tail) {
-                    const int pos = e & (FQ - 1);
-                    const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
-                    const int r = fin_draw(par[gg], ii);
-                    if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
-                    if (r & 2) atomicAdd(&alt_s[w][gg], 1);
-                }
-                head = min(head + 64, tail);
-                wave_sync();
-            }
-        }
-        // the last flush may leave entries when more than 64 were pending
-        while (tail > head) {   // wave-uniform
-            wave_sync();
-            const int e = head + lane;
-            if (e < tail) {
-                const int pos = e & (FQ - 1);
-                const int gg = qi[pos] >> 9, ii = qi[pos] & 511;
-                const int r = fin_draw(par[gg], ii);
+        const uint64_t ua = (uint64_t)(2 * par[g].axis + 1) * GOLDEN_GAMMA;
+        // Stage 1 hashes every draw's screening uniform and queues the indices (and uniforms) of the draws
+        // that pass; stage 2 takes 64 queued draws at a time, hashes their second uniform and queues those
+        // whose FP32 z_k estimate lies in I_k; stage 3 evaluates 64 of these at a time (fin_draw).  The
+        // queues are per-wavefront LDS rings of FQ entries with wave-uniform head and tail.
+        int h1 = 0, t1 = 0, h2 = 0, t2 = 0;
+        auto stage3 = [&]() {
+            const int e3 = h2 + lane;
+            if (e3 < t2) {
+                const unsigned short id3 = qx[e3 & (FQ - 1)];
+                const int gg = id3 >> 9;
+                const int r = fin_draw(par[gg], id3 & 511);
                 if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
                 if (r & 2) atomicAdd(&alt_s[w][gg], 1);
             }
-            head = min(head + 64, tail);
+            h2 = min(h2 + 64, t2);
             wave_sync();
+        };
+        // splitmix64 state before draw i's first uniform: seed + 6 i gamma, advanced by 48 gamma per step
+        uint64_t st = seed + (uint64_t)(6 * p) * GOLDEN_GAMMA + ua;
+        for (int t = 0; t <= 64; t++, st += 48 * GOLDEN_GAMMA) {
+            if (t < 64) {   // stage 1
+                const uint64_t w1 = mix64(st) >> 11;
+                const bool pass = !none && w1 <= u1max;   // a draw whose z_k lies outside I_k cannot count
+                const unsigned long long bal = __ballot(pass);
+                if (pass) {
+                    const int pos = (t1 + __popcll(bal & below)) & (FQ - 1);
+                    qw[pos] = w1;
+                    qi[pos] = (unsigned short)(g << 9 | (p + 8 * t));
+                }
+                t1 += __popcll(bal);
+            }
+            while (t1 - h1 >= 64 || (t == 64 && t1 > h1)) {   // stage 2, wave-uniform
+                wave_sync();
+                const int e = h1 + lane;
+                bool pass = false;
+                unsigned short id = 0;
+                if (e < t1) {
+                    const int pos = e & (FQ - 1);
+                    id = qi[pos];
+                    const FinPar &f = par[id >> 9];
+                    const uint64_t b = f.seed + (uint64_t)(6 * (id & 511) + 2 * f.axis + 2) * GOLDEN_GAMMA;
+                    const double u1 = (double)qw[pos] * (1.0 / 9007199254740992.0);
+                    const float ze = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, to_unit(mix64(b)));
+                    pass = f.all || (ze >= f.flo && ze <= f.fhi);
+                }
+                const unsigned long long bal = __ballot(pass);
+                if (pass) qx[(t2 + __popcll(bal & below)) & (FQ - 1)] = id;
+                t2 += __popcll(bal);
+                h1 = min(h1 + 64, t1);
+                wave_sync();
+                while (t2 - h2 >= 64) stage3();   // wave-uniform
+            }
         }
+        while (t2 > h2) stage3();   // wave-uniform
         wave_sync();
         int in50, intot;
         const int cbig = cnt_s[w][g], csmall = alt_s[w][g];

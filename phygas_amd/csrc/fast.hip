@@ -725,7 +725,38 @@ __device__ int fin_draw(const FinPar &f, int i) {
 // and the exact path run on full wavefronts.  Pairs
 // of the p_min mode whose r50 count reaches n50_min have their r_tot count taken by the group (rare).
 constexpr int FQ = 128;   // queue ring per wavefront
-__global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ kept,
+
+// Frame of the draws of a kept pair (star s, cluster c, point kb, segment parameter sb): the mean m (the
+// point of the step's segment), the covariance C = S_s + S_c at kb and its Cholesky factor L
+// (l00 l10 l20 l11 l21 l22), the seed of the pair's generator.  One function for the finish and for the
+// r_tot counts of its rows, so both see the same bits.
+__device__ __forceinline__ void pair_frame(const double *__restrict__ strk, const double *__restrict__ ctrk,
+                                           const int *__restrict__ cn, const PhCluster *__restrict__ cl,
+                                           const uint64_t *__restrict__ sprefix, int npts, int s, int c, int kb,
+                                           double sb, V3 &m, double C[6], double L[6], uint64_t &seed) {
+    const int n = min(npts, cn[c]);
+    const int k1 = kb + 1 < n ? kb + 1 : kb;
+    const double *ps = strk + (size_t)s * npts * 12, *pc = ctrk + (size_t)c * npts * 12;
+    const V3 a = v3(ps[kb * 12] - pc[kb * 12], ps[kb * 12 + 1] - pc[kb * 12 + 1], ps[kb * 12 + 2] - pc[kb * 12 + 2]);
+    const V3 b = v3(ps[k1 * 12] - pc[k1 * 12], ps[k1 * 12 + 1] - pc[k1 * 12 + 1], ps[k1 * 12 + 2] - pc[k1 * 12 + 2]);
+    m = a + (b - a) * sb;
+#pragma unroll
+    for (int i = 0; i < 6; i++) C[i] = ps[kb * 12 + 6 + i] + pc[kb * 12 + 6 + i];
+    L[0] = sqrt(fmax(C[0], 0.0));
+    L[1] = L[0] > 0 ? C[1] / L[0] : 0;
+    L[2] = L[0] > 0 ? C[2] / L[0] : 0;
+    L[3] = sqrt(fmax(C[3] - L[1] * L[1], 0.0));
+    L[4] = L[3] > 0 ? (C[4] - L[2] * L[1]) / L[3] : 0;
+    L[5] = sqrt(fmax(C[5] - L[2] * L[2] - L[4] * L[4], 0.0));
+    seed = fnv_feed(sprefix[s], cl[c].name, 40);
+}
+
+// What a pair's row needs after the draws (kept in LDS across the draw loop).
+struct FinRow {
+    int s, c, kb, alt_is_tot;
+    double chi2, sb, dnom, sig;
+};
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_fast(const Kept *__restrict__ kept,
                                                      const unsigned long long *__restrict__ nkept, long kept_cap,
                                                      const double *__restrict__ strk, const double *__restrict__ ctrk,
                                                      const int *__restrict__ cn, const double *__restrict__ r50s,
@@ -739,8 +770,10 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
     __shared__ uint64_t qw_s[4][FQ];
     __shared__ unsigned short qi_s[4][FQ], qx_s[4][FQ];
     __shared__ int cnt_s[4][8], alt_s[4][8];
+    __shared__ FinRow post_s[4][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
     FinPar *par = par_s[w];
+    FinRow *post = post_s[w];
     uint64_t *qw = qw_s[w];
     unsigned short *qi = qi_s[w], *qx = qx_s[w];
     const long nk = min((long)*nkept, kept_cap);
@@ -750,31 +783,19 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         const long h = h0 + g;
         const bool live = h < nk;
         const Kept kp = kept[live ? h : h0];
-        const int s = kp.star, c = kp.cluster, kb = kp.kb;
-        const int n = min(npts, cn[c]);
-        const int k1 = kb + 1 < n ? kb + 1 : kb;
-        const double *ps = strk + (size_t)s * npts * 12, *pc = ctrk + (size_t)c * npts * 12;
-        const V3 a = v3(ps[kb * 12] - pc[kb * 12], ps[kb * 12 + 1] - pc[kb * 12 + 1], ps[kb * 12 + 2] - pc[kb * 12 + 2]);
-        const V3 b = v3(ps[k1 * 12] - pc[k1 * 12], ps[k1 * 12 + 1] - pc[k1 * 12 + 1], ps[k1 * 12 + 2] - pc[k1 * 12 + 2]);
-        const V3 m = a + (b - a) * kp.sb;
-        double C[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) C[i] = ps[kb * 12 + 6 + i] + pc[kb * 12 + 6 + i];
-        double L[6];
-        L[0] = sqrt(fmax(C[0], 0.0));
-        L[1] = L[0] > 0 ? C[1] / L[0] : 0;
-        L[2] = L[0] > 0 ? C[2] / L[0] : 0;
-        L[3] = sqrt(fmax(C[3] - L[1] * L[1], 0.0));
-        L[4] = L[3] > 0 ? (C[4] - L[2] * L[1]) / L[3] : 0;
-        L[5] = sqrt(fmax(C[5] - L[2] * L[2] - L[4] * L[4], 0.0));
-        const uint64_t seed = fnv_feed(sprefix[s], cl[c].name, 40);
-        const double r50 = r50s[c], rt = rts[c];
-        const double rbig = n50_min > 0 ? r50 : fmax(rt, r50), rsmall = n50_min > 0 ? r50 : (rt < r50 ? rt : r50);
+        V3 m;
+        double C[6], L[6];
+        uint64_t seed;
+        pair_frame(strk, ctrk, cn, cl, sprefix, npts, kp.star, kp.cluster, kp.kb, kp.sb, m, C, L, seed);
         if (p == 0) {
+            const double r50 = r50s[kp.cluster], rt = rts[kp.cluster];
+            const double rbig = n50_min > 0 ? r50 : fmax(rt, r50), rsmall = n50_min > 0 ? r50 : (rt < r50 ? rt : r50);
             fin_setup(par[g], seed, L, m, rbig, rsmall);
             if (!live) par[g].none = 1;
             cnt_s[w][g] = 0;
             alt_s[w][g] = 0;
+            post[g] = FinRow{kp.star, kp.cluster, kp.kb, rt < r50 ? 1 : 0, kp.chi2, kp.sb, norm(m) * 1000.0,
+                             sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0};
         }
         wave_sync();
         const bool none = par[g].none;
@@ -835,33 +856,52 @@ __global__ __launch_bounds__(256) void k_finish_fast(const Kept *__restrict__ ke
         }
         while (t2 > h2) stage3();   // wave-uniform
         wave_sync();
-        int in50, intot;
         const int cbig = cnt_s[w][g], csmall = alt_s[w][g];
-        bool row = live;
-        if (n50_min > 0) {
-            in50 = cbig;
-            row = row && in50 >= n50_min;
-            intot = 0;
-            if (row) intot = count_inside8(seed, L, m, rt, rt, nullptr, p);   // group-uniform, rare
-        } else {
-            if (rt < r50) { intot = csmall; in50 = cbig; } else { intot = cbig; in50 = csmall; }
-        }
+        const FinRow fr = post[g];
+        // p_min mode: rows need in50 >= n50_min; their r_tot counts are taken by k_rows_rtot
+        const bool row = live && (n50_min <= 0 || cbig >= n50_min);
+        const int in50 = (n50_min > 0 || fr.alt_is_tot) ? cbig : csmall;
+        const int intot = n50_min > 0 ? 0 : (fr.alt_is_tot ? csmall : cbig);
         if (row && p == 0) {
+            const double *ps = strk + (size_t)fr.s * npts * 12, *pc = ctrk + (size_t)fr.c * npts * 12;
             PhxRow o;
-            o.star = s; o.cluster = c; o.kb = kb; o.flags = sgroup[s];
-            o.chi2 = kp.chi2; o.sb = kp.sb;
-            o.t = (kb + kp.sb) * STEP_MYR;
-            o.d_nom_pc = norm(m) * 1000.0;
-            o.sig_pc = sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0;
+            o.star = fr.s; o.cluster = fr.c; o.kb = fr.kb; o.flags = sgroup[fr.s];
+            o.chi2 = fr.chi2; o.sb = fr.sb;
+            o.t = (fr.kb + fr.sb) * STEP_MYR;
+            o.d_nom_pc = fr.dnom;
+            o.sig_pc = fr.sig;
             o.p_r50 = (double)in50 / 512;
             o.p_rtot = (double)intot / 512;
-            const V3 dv = v3(ps[kb * 12 + 3] - pc[kb * 12 + 3], ps[kb * 12 + 4] - pc[kb * 12 + 4],
-                             ps[kb * 12 + 5] - pc[kb * 12 + 5]);
+            const V3 dv = v3(ps[fr.kb * 12 + 3] - pc[fr.kb * 12 + 3], ps[fr.kb * 12 + 4] - pc[fr.kb * 12 + 4],
+                             ps[fr.kb * 12 + 5] - pc[fr.kb * 12 + 5]);
             o.vrel = norm(dv) * KMS_PER_KPCMYR;
             const unsigned long long ro = atomicAdd(nrows, 1ull);
             if ((long)ro < row_cap) rows[ro] = o;
         }
         wave_sync();   // par / counters are rewritten by the next batch
+    }
+}
+
+// r_tot counts of the rows of a p_min pass: eight lanes per row (draws p, p + 8, ...), the pair's frame
+// rebuilt from the row as the finish built it.
+__global__ __launch_bounds__(256) void k_rows_rtot(const double *__restrict__ strk, const double *__restrict__ ctrk,
+                                                   const int *__restrict__ cn, const double *__restrict__ rts,
+                                                   const PhCluster *__restrict__ cl,
+                                                   const uint64_t *__restrict__ sprefix, int npts,
+                                                   PhxRow *__restrict__ rows,
+                                                   const unsigned long long *__restrict__ nrows, long row_cap) {
+    const long n = min((long)*nrows, row_cap);
+    const int p = threadIdx.x & 7;
+    for (long r0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3; r0 < n;
+         r0 += ((long)gridDim.x * blockDim.x) >> 3) {   // group-uniform
+        const PhxRow o = rows[r0];
+        V3 m;
+        double C[6], L[6];
+        uint64_t seed;
+        pair_frame(strk, ctrk, cn, cl, sprefix, npts, o.star, o.cluster, o.kb, o.sb, m, C, L, seed);
+        const double rt = rts[o.cluster];
+        const int intot = count_inside8(seed, L, m, rt, rt, nullptr, p);
+        if (p == 0) rows[r0].p_rtot = (double)intot / 512;
     }
 }
 
@@ -1019,6 +1059,11 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
                        trk, (const double *)T.mem[T_CTRK], a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min,
                        (PhxRow *)W.mem[W_ROWS], cnt + C_ROWS, W.row_cap);
     PHX_CK(hipGetLastError());
+    if (n50_min > 0) {
+        hipLaunchKernelGGL(k_rows_rtot, dim3(256), dim3(256), 0, st, trk, (const double *)T.mem[T_CTRK], a.cn, a.rt,
+                           a.cl, a.prefix, npts, (PhxRow *)W.mem[W_ROWS], cnt + C_ROWS, W.row_cap);
+        PHX_CK(hipGetLastError());
+    }
     if (mark) mark(mark_ctx, "finish");
     return 0;
 }

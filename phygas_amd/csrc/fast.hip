@@ -56,9 +56,10 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
-       W_TILED, W_NSLOT };
-// Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_N };
+       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_NSLOT };
+// Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
+// for the exact evaluation by the finish screen
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_N };
 // Table buffers (FastTables::mem slots)
 enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_NSLOT };
 
@@ -634,12 +635,16 @@ __device__ int count_inside8(uint64_t seed, const double l[6], V3 m, double r, d
     return cnt;
 }
 
-// Per-pair frame of the finish, shared by the lanes of a wavefront through LDS.
-struct FinPar {
+// Per kept pair: the draw screen (FinScr) and the frame of the exact evaluation of a draw (FinDraw).
+struct FinScr {
     uint64_t seed, u1max;
-    double mx, my, mz, l[6], r, ra, marg;
     float flo, fhi;
-    int all, none, axis;
+    int flags;   // bit 0: every draw passes the screen, bit 1: no draw can count, bits 2-3: the axis k
+    int pad;
+};
+struct FinDraw {
+    uint64_t seed;
+    double mx, my, mz, l[6], r, ra, marg;
 };
 
 // Screens for one radius pair (r, ra <= r).  A draw counts only when z = L^-1 (x - m) lies in the ellipsoid
@@ -648,8 +653,9 @@ struct FinPar {
 // uniform u: a draw whose u exceeds exp(-dist(0, I_k)^2 / 2) cannot count.  The axis with the farthest
 // interval screens (one hash per draw for most draws), then the FP32 estimate of z_k against I_k.  With a
 // singular L only the x axis is used (I_0 from |m.x + l00 z0| < r).  none = 1: no draw can count.
-__device__ void fin_setup(FinPar &f, uint64_t seed, const double L[6], V3 m, double r, double ra) {
+__device__ void fin_setup(FinScr &q, FinDraw &f, uint64_t seed, const double L[6], V3 m, double r, double ra) {
     f.seed = seed;
+    q.seed = seed;
     f.mx = m.x; f.my = m.y; f.mz = m.z;
 #pragma unroll
     for (int i = 0; i < 6; i++) f.l[i] = L[i];
@@ -686,18 +692,18 @@ __device__ void fin_setup(FinPar &f, uint64_t seed, const double L[6], V3 m, dou
         hi = 1e300;
     }
     // |z_k| <= 9.42 for every representable u; with l00 = 0, x0 = m.x for every draw
-    f.none = (!all && (lo >= 9.5 || hi <= -9.5)) || (L[0] <= 0 && !all);
-    f.all = all;
-    f.axis = axis;
+    const bool none = (!all && (lo >= 9.5 || hi <= -9.5)) || (L[0] <= 0 && !all);
+    q.flags = (all ? 1 : 0) | (none ? 2 : 0) | axis << 2;
+    q.pad = 0;
     const double a = all ? 0.0 : (lo > 0 ? lo : (hi < 0 ? -hi : 0.0));
     const double ucut = exp(-0.5 * a * a * (1.0 - 1e-9)) * (1.0 + 1e-9);   // u above this: |z_k| < a
-    f.u1max = ucut >= 1.0 ? ~0ull : (uint64_t)(ucut * 9007199254740992.0) + 1;
-    f.flo = (float)lo - 1e-3f;
-    f.fhi = (float)hi + 1e-3f;
+    q.u1max = ucut >= 1.0 ? ~0ull : (uint64_t)(ucut * 9007199254740992.0) + 1;
+    q.flo = (float)lo - 1e-3f;
+    q.fhi = (float)hi + 1e-3f;
 }
 
 // Draw i of a pair that passed the axis screen: bit 0 when it counts for r, bit 1 for ra.
-__device__ int fin_draw(const FinPar &f, int i) {
+__device__ int fin_draw(const FinDraw &f, int i) {
     const uint64_t b = f.seed + (uint64_t)(6 * i) * GOLDEN_GAMMA;
     const double u1 = to_unit(mix64(b + GOLDEN_GAMMA)), u2 = to_unit(mix64(b + 2 * GOLDEN_GAMMA));
     const double u3 = to_unit(mix64(b + 3 * GOLDEN_GAMMA)), u4 = to_unit(mix64(b + 4 * GOLDEN_GAMMA));
@@ -719,17 +725,10 @@ __device__ int fin_draw(const FinPar &f, int i) {
     return (rr < f.r ? 1 : 0) | (rr < f.ra ? 2 : 0);
 }
 
-// One wavefront per 8 kept pairs.  Lane p of group g screens draws p, p+8, ... of pair g (512 per pair):
-// u1 against the integer bound, then the FP32 z0 estimate against the x interval; the few draws left go to
-// a wave-local queue in LDS, evaluated 64 entries at a time (one per lane), so that the whole-draw estimate
-// and the exact path run on full wavefronts.  Pairs
-// of the p_min mode whose r50 count reaches n50_min have their r_tot count taken by the group (rare).
-constexpr int FQ = 128;   // queue ring per wavefront
-
 // Frame of the draws of a kept pair (star s, cluster c, point kb, segment parameter sb): the mean m (the
 // point of the step's segment), the covariance C = S_s + S_c at kb and its Cholesky factor L
-// (l00 l10 l20 l11 l21 l22), the seed of the pair's generator.  One function for the finish and for the
-// r_tot counts of its rows, so both see the same bits.
+// (l00 l10 l20 l11 l21 l22), the seed of the pair's generator.  One function for every kernel that needs
+// it, so all see the same bits.
 __device__ __forceinline__ void pair_frame(const double *__restrict__ strk, const double *__restrict__ ctrk,
                                            const int *__restrict__ cn, const PhCluster *__restrict__ cl,
                                            const uint64_t *__restrict__ sprefix, int npts, int s, int c, int kb,
@@ -751,75 +750,85 @@ __device__ __forceinline__ void pair_frame(const double *__restrict__ strk, cons
     seed = fnv_feed(sprefix[s], cl[c].name, 40);
 }
 
-// What a pair's row needs after the draws (kept in LDS across the draw loop).
-struct FinRow {
-    int s, c, kb, alt_is_tot;
-    double chi2, sb, dnom, sig;
-};
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_finish_fast(const Kept *__restrict__ kept,
-                                                     const unsigned long long *__restrict__ nkept, long kept_cap,
-                                                     const double *__restrict__ strk, const double *__restrict__ ctrk,
-                                                     const int *__restrict__ cn, const double *__restrict__ r50s,
-                                                     const double *__restrict__ rts,
-                                                     const PhCluster *__restrict__ cl,
-                                                     const uint64_t *__restrict__ sprefix,
-                                                     const int *__restrict__ sgroup, int npts, int n50_min,
-                                                     PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
-                                                     long row_cap) {
-    __shared__ FinPar par_s[4][8];
-    __shared__ uint64_t qw_s[4][FQ];
-    __shared__ unsigned short qi_s[4][FQ], qx_s[4][FQ];
-    __shared__ int cnt_s[4][8], alt_s[4][8];
-    __shared__ FinRow post_s[4][8];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
-    FinPar *par = par_s[w];
-    FinRow *post = post_s[w];
-    uint64_t *qw = qw_s[w];
-    unsigned short *qi = qi_s[w], *qx = qx_s[w];
+// The finish runs as four kernels so that the draw loop, which is most of its work, keeps few registers
+// (and many wavefronts per SIMD):
+//   k_fin_setup    one lane per kept pair: its frame, screen and counters (zeroed)
+//   k_finish_fast  the draw screen (below); the draws it cannot decide go to one list
+//   k_fin_draw     one lane per listed draw: fin_draw, counted with atomics into the pair's counters
+//   k_fin_rows     one lane per kept pair: the row when the counts make one (frame rebuilt for rows only)
+// and k_rows_rtot takes the r_tot counts of the rows of a p_min pass.
+//
+// Counting radii: r = max(r50, r_tot) and ra = min(r50, r_tot) (both r50 in the p_min mode).
+__global__ __launch_bounds__(256) void k_fin_setup(const Kept *__restrict__ kept,
+                                                   const unsigned long long *__restrict__ nkept, long kept_cap,
+                                                   const double *__restrict__ strk, const double *__restrict__ ctrk,
+                                                   const int *__restrict__ cn, const double *__restrict__ r50s,
+                                                   const double *__restrict__ rts, const PhCluster *__restrict__ cl,
+                                                   const uint64_t *__restrict__ sprefix, int npts, int n50_min,
+                                                   FinScr *__restrict__ fscr, FinDraw *__restrict__ fdraw,
+                                                   int2 *__restrict__ fcnt) {
     const long nk = min((long)*nkept, kept_cap);
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (long h0 = wave * 8; h0 < nk; h0 += nwave * 8) {   // wave-uniform
-        const long h = h0 + g;
-        const bool live = h < nk;
-        const Kept kp = kept[live ? h : h0];
+    for (long h = (long)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (long)gridDim.x * blockDim.x) {
+        const Kept kp = kept[h];
         V3 m;
         double C[6], L[6];
         uint64_t seed;
         pair_frame(strk, ctrk, cn, cl, sprefix, npts, kp.star, kp.cluster, kp.kb, kp.sb, m, C, L, seed);
-        if (p == 0) {
-            const double r50 = r50s[kp.cluster], rt = rts[kp.cluster];
-            const double rbig = n50_min > 0 ? r50 : fmax(rt, r50), rsmall = n50_min > 0 ? r50 : (rt < r50 ? rt : r50);
-            fin_setup(par[g], seed, L, m, rbig, rsmall);
-            if (!live) par[g].none = 1;
-            cnt_s[w][g] = 0;
-            alt_s[w][g] = 0;
-            post[g] = FinRow{kp.star, kp.cluster, kp.kb, rt < r50 ? 1 : 0, kp.chi2, kp.sb, norm(m) * 1000.0,
-                             sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0};
-        }
+        const double r50 = r50s[kp.cluster], rt = rts[kp.cluster];
+        const double rbig = n50_min > 0 ? r50 : fmax(rt, r50), rsmall = n50_min > 0 ? r50 : (rt < r50 ? rt : r50);
+        FinScr q;
+        FinDraw f;
+        fin_setup(q, f, seed, L, m, rbig, rsmall);
+        fscr[h] = q;
+        fdraw[h] = f;
+        fcnt[h] = make_int2(0, 0);
+    }
+}
+
+// The draw screen: one wavefront per 8 kept pairs, lane p of group g takes draws p, p + 8, ... of pair g.
+// Stage 1 hashes every draw's screening uniform (u of axis k) and queues the draws that pass the integer
+// bound; stage 2 takes 64 queued draws at a time, hashes their second uniform and sends those whose FP32 z_k
+// estimate lies in I_k (+- 1e-3) to the list of draws for k_fin_draw.  Both queues are per-wavefront LDS
+// rings of FQ entries with wave-uniform heads and tails; the list takes one atomic per 64 draws.
+constexpr int FQ = 128;
+__global__ __launch_bounds__(256) void k_finish_fast(const FinScr *__restrict__ fscr,
+                                                     const unsigned long long *__restrict__ nkept, long kept_cap,
+                                                     uint64_t *__restrict__ surv, unsigned long long *__restrict__ nsurv,
+                                                     long surv_cap) {
+    __shared__ FinScr par_s[4][8];
+    __shared__ uint64_t qw_s[4][FQ];
+    __shared__ uint64_t qx_s[4][FQ];
+    __shared__ unsigned short qi_s[4][FQ];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
+    FinScr *par = par_s[w];
+    uint64_t *qw = qw_s[w], *qx = qx_s[w];
+    unsigned short *qi = qi_s[w];
+    const long nk = min((long)*nkept, kept_cap);
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // draws for k_fin_draw wait in a second ring (qx, head h2, tail t2) and go to the list 64 at a time
+    int h2 = 0, t2 = 0;
+    auto flush = [&]() {
+        const int nf = min(t2 - h2, 64);
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(nsurv, (unsigned long long)nf);
+        base = __shfl(base, 0, 64);
+        if (lane < nf && (long)(base + lane) < surv_cap) surv[base + lane] = qx[(h2 + lane) & (FQ - 1)];
+        h2 += nf;
         wave_sync();
-        const bool none = par[g].none;
-        const uint64_t u1max = par[g].u1max;
-        const uint64_t ua = (uint64_t)(2 * par[g].axis + 1) * GOLDEN_GAMMA;
-        // Stage 1 hashes every draw's screening uniform and queues the indices (and uniforms) of the draws
-        // that pass; stage 2 takes 64 queued draws at a time, hashes their second uniform and queues those
-        // whose FP32 z_k estimate lies in I_k; stage 3 evaluates 64 of these at a time (fin_draw).  The
-        // queues are per-wavefront LDS rings of FQ entries with wave-uniform head and tail.
-        int h1 = 0, t1 = 0, h2 = 0, t2 = 0;
-        auto stage3 = [&]() {
-            const int e3 = h2 + lane;
-            if (e3 < t2) {
-                const unsigned short id3 = qx[e3 & (FQ - 1)];
-                const int gg = id3 >> 9;
-                const int r = fin_draw(par[gg], id3 & 511);
-                if (r & 1) atomicAdd(&cnt_s[w][gg], 1);
-                if (r & 2) atomicAdd(&alt_s[w][gg], 1);
-            }
-            h2 = min(h2 + 64, t2);
-            wave_sync();
-        };
-        // splitmix64 state before draw i's first uniform: seed + 6 i gamma, advanced by 48 gamma per step
-        uint64_t st = seed + (uint64_t)(6 * p) * GOLDEN_GAMMA + ua;
+    };
+    for (long h0 = wave * 8; h0 < nk; h0 += nwave * 8) {   // wave-uniform
+        const long h = h0 + g;
+        const bool live = h < nk;
+        FinScr q = fscr[live ? h : h0];
+        if (!live) q.flags = 2;
+        if (p == 0) par[g] = q;
+        const bool none = (q.flags & 2) != 0;
+        const uint64_t u1max = q.u1max;
+        int h1 = 0, t1 = 0;
+        // splitmix64 state before draw i's screening uniform: seed + (6 i + 2 k + 1) gamma, +48 gamma per step
+        uint64_t st = q.seed + (uint64_t)(6 * p + 2 * (q.flags >> 2) + 1) * GOLDEN_GAMMA;
+        wave_sync();
         for (int t = 0; t <= 64; t++, st += 48 * GOLDEN_GAMMA) {
             if (t < 64) {   // stage 1
                 const uint64_t w1 = mix64(st) >> 11;
@@ -836,49 +845,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 wave_sync();
                 const int e = h1 + lane;
                 bool pass = false;
-                unsigned short id = 0;
+                uint64_t id = 0;
                 if (e < t1) {
                     const int pos = e & (FQ - 1);
-                    id = qi[pos];
-                    const FinPar &f = par[id >> 9];
-                    const uint64_t b = f.seed + (uint64_t)(6 * (id & 511) + 2 * f.axis + 2) * GOLDEN_GAMMA;
+                    const int v = qi[pos], gg = v >> 9, i = v & 511;
+                    const FinScr &f = par[gg];
+                    const uint64_t b = f.seed + (uint64_t)(6 * i + 2 * (f.flags >> 2) + 2) * GOLDEN_GAMMA;
                     const double u1 = (double)qw[pos] * (1.0 / 9007199254740992.0);
                     const float ze = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, to_unit(mix64(b)));
-                    pass = f.all || (ze >= f.flo && ze <= f.fhi);
+                    pass = (f.flags & 1) || (ze >= f.flo && ze <= f.fhi);
+                    id = (uint64_t)(h0 + gg) << 9 | (uint64_t)i;
                 }
                 const unsigned long long bal = __ballot(pass);
                 if (pass) qx[(t2 + __popcll(bal & below)) & (FQ - 1)] = id;
                 t2 += __popcll(bal);
                 h1 = min(h1 + 64, t1);
                 wave_sync();
-                while (t2 - h2 >= 64) stage3();   // wave-uniform
+                if (t2 - h2 >= 64) flush();   // wave-uniform
             }
         }
-        while (t2 > h2) stage3();   // wave-uniform
-        wave_sync();
-        const int cbig = cnt_s[w][g], csmall = alt_s[w][g];
-        const FinRow fr = post[g];
-        // p_min mode: rows need in50 >= n50_min; their r_tot counts are taken by k_rows_rtot
-        const bool row = live && (n50_min <= 0 || cbig >= n50_min);
-        const int in50 = (n50_min > 0 || fr.alt_is_tot) ? cbig : csmall;
-        const int intot = n50_min > 0 ? 0 : (fr.alt_is_tot ? csmall : cbig);
-        if (row && p == 0) {
-            const double *ps = strk + (size_t)fr.s * npts * 12, *pc = ctrk + (size_t)fr.c * npts * 12;
-            PhxRow o;
-            o.star = fr.s; o.cluster = fr.c; o.kb = fr.kb; o.flags = sgroup[fr.s];
-            o.chi2 = fr.chi2; o.sb = fr.sb;
-            o.t = (fr.kb + fr.sb) * STEP_MYR;
-            o.d_nom_pc = fr.dnom;
-            o.sig_pc = fr.sig;
-            o.p_r50 = (double)in50 / 512;
-            o.p_rtot = (double)intot / 512;
-            const V3 dv = v3(ps[fr.kb * 12 + 3] - pc[fr.kb * 12 + 3], ps[fr.kb * 12 + 4] - pc[fr.kb * 12 + 4],
-                             ps[fr.kb * 12 + 5] - pc[fr.kb * 12 + 5]);
-            o.vrel = norm(dv) * KMS_PER_KPCMYR;
-            const unsigned long long ro = atomicAdd(nrows, 1ull);
-            if ((long)ro < row_cap) rows[ro] = o;
-        }
-        wave_sync();   // par / counters are rewritten by the next batch
+        wave_sync();   // par is rewritten by the next batch
+    }
+    if (t2 > h2) flush();   // wave-uniform
+}
+
+// One lane per listed draw (pair h = id >> 9, draw id & 511): the whole-draw estimate, else the exact draw.
+__global__ __launch_bounds__(256) void k_fin_draw(const uint64_t *__restrict__ surv,
+                                                  const unsigned long long *__restrict__ nsurv, long surv_cap,
+                                                  const FinDraw *__restrict__ fdraw, int2 *__restrict__ fcnt) {
+    const long n = (long)*nsurv;
+    if (n > surv_cap) return;   // the pass is run again with a longer list
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const uint64_t id = surv[e];
+        const long h = (long)(id >> 9);
+        const int r = fin_draw(fdraw[h], (int)(id & 511));
+        if (r & 1) atomicAdd(&fcnt[h].x, 1);
+        if (r & 2) atomicAdd(&fcnt[h].y, 1);
+    }
+}
+
+// One lane per kept pair: its row when the counts make one (every pair without p_min; in50 >= n50_min with it,
+// the r_tot count then left to k_rows_rtot).
+__global__ __launch_bounds__(256) void k_fin_rows(const Kept *__restrict__ kept,
+                                                  const unsigned long long *__restrict__ nkept, long kept_cap,
+                                                  const int2 *__restrict__ fcnt, const double *__restrict__ strk,
+                                                  const double *__restrict__ ctrk, const int *__restrict__ cn,
+                                                  const double *__restrict__ r50s, const double *__restrict__ rts,
+                                                  const PhCluster *__restrict__ cl,
+                                                  const uint64_t *__restrict__ sprefix,
+                                                  const int *__restrict__ sgroup, int npts, int n50_min,
+                                                  PhxRow *__restrict__ rows, unsigned long long *__restrict__ nrows,
+                                                  long row_cap) {
+    const long nk = min((long)*nkept, kept_cap);
+    for (long h = (long)blockIdx.x * blockDim.x + threadIdx.x; h < nk; h += (long)gridDim.x * blockDim.x) {
+        const int2 k = fcnt[h];
+        const int cbig = k.x, csmall = k.y;
+        if (n50_min > 0 && cbig < n50_min) continue;
+        const Kept kp = kept[h];
+        const double r50 = r50s[kp.cluster], rt = rts[kp.cluster];
+        int in50, intot;
+        if (n50_min > 0) { in50 = cbig; intot = 0; }
+        else if (rt < r50) { intot = csmall; in50 = cbig; }
+        else { intot = cbig; in50 = csmall; }
+        V3 m;
+        double C[6], L[6];
+        uint64_t seed;
+        pair_frame(strk, ctrk, cn, cl, sprefix, npts, kp.star, kp.cluster, kp.kb, kp.sb, m, C, L, seed);
+        const double *ps = strk + (size_t)kp.star * npts * 12, *pc = ctrk + (size_t)kp.cluster * npts * 12;
+        PhxRow o;
+        o.star = kp.star; o.cluster = kp.cluster; o.kb = kp.kb; o.flags = sgroup[kp.star];
+        o.chi2 = kp.chi2; o.sb = kp.sb;
+        o.t = (kp.kb + kp.sb) * STEP_MYR;
+        o.d_nom_pc = norm(m) * 1000.0;
+        o.sig_pc = sqrt((C[0] + C[3] + C[5]) / 3.0) * 1000.0;
+        o.p_r50 = (double)in50 / 512;
+        o.p_rtot = (double)intot / 512;
+        const V3 dv = v3(ps[kp.kb * 12 + 3] - pc[kp.kb * 12 + 3], ps[kp.kb * 12 + 4] - pc[kp.kb * 12 + 4],
+                         ps[kp.kb * 12 + 5] - pc[kp.kb * 12 + 5]);
+        o.vrel = norm(dv) * KMS_PER_KPCMYR;
+        const unsigned long long ro = atomicAdd(nrows, 1ull);
+        if ((long)ro < row_cap) rows[ro] = o;
     }
 }
 
@@ -985,6 +1031,7 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
     if (W.kept_cap == 0) W.kept_cap = std::max(1L << 18, npairs / 16);
     if (W.row_cap == 0) W.row_cap = std::max(1L << 16, W.kept_cap / 8);
     if (W.item_cap == 0) W.item_cap = std::max(1L << 22, npairs);
+    if (W.surv_cap == 0) W.surv_cap = std::max(1L << 20, 4 * W.kept_cap);
     if (ensure(W.mem[W_PS0], W.bytes[W_PS0], 48 * (size_t)ns) || ensure(W.mem[W_SIG], W.bytes[W_SIG], 288 * (size_t)ns) ||
         ensure(W.mem[W_TRK], W.bytes[W_TRK], sizeof(double) * 12 * (size_t)ns * npts) ||
         ensure(W.mem[W_TILED], W.bytes[W_TILED], sizeof(double) * TREC * (size_t)ns * ntl) ||
@@ -996,6 +1043,10 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
         ensure(W.mem[W_ITEMS], W.bytes[W_ITEMS], sizeof(Item) * (size_t)W.item_cap) ||
         ensure(W.mem[W_IRES], W.bytes[W_IRES], sizeof(IRes) * (size_t)W.item_cap) ||
         ensure(W.mem[W_ISTART], W.bytes[W_ISTART], sizeof(long) * (size_t)W.cand_cap) ||
+        ensure(W.mem[W_FSCR], W.bytes[W_FSCR], sizeof(FinScr) * (size_t)W.kept_cap) ||
+        ensure(W.mem[W_FDRAW], W.bytes[W_FDRAW], sizeof(FinDraw) * (size_t)W.kept_cap) ||
+        ensure(W.mem[W_FCNT], W.bytes[W_FCNT], sizeof(int2) * (size_t)W.kept_cap) ||
+        ensure(W.mem[W_SURV], W.bytes[W_SURV], sizeof(uint64_t) * (size_t)W.surv_cap) ||
 
         ensure(W.mem[W_CNT], W.bytes[W_CNT], C_N * sizeof(unsigned long long)))
         return 1;
@@ -1055,9 +1106,25 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "reduce");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;
-    hipLaunchKernelGGL(k_finish_fast, dim3(W.finish_blocks > 0 ? W.finish_blocks : 4096), dim3(256), 0, st, (const Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap,
-                       trk, (const double *)T.mem[T_CTRK], a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min,
-                       (PhxRow *)W.mem[W_ROWS], cnt + C_ROWS, W.row_cap);
+    const Kept *kept = (const Kept *)W.mem[W_KEPT];
+    FinScr *fscr = (FinScr *)W.mem[W_FSCR];
+    FinDraw *fdraw = (FinDraw *)W.mem[W_FDRAW];
+    int2 *fcnt = (int2 *)W.mem[W_FCNT];
+    const double *ctrk = (const double *)T.mem[T_CTRK];
+    const unsigned kgrid = (unsigned)std::min<long>(blocks_for(W.kept_cap, 256), 8192);
+    hipLaunchKernelGGL(k_fin_setup, dim3(kgrid), dim3(256), 0, st, kept, cnt + C_KEPT, W.kept_cap, trk, ctrk, a.cn,
+                       a.r50, a.rt, a.cl, a.prefix, npts, n50_min, fscr, fdraw, fcnt);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_fast, dim3(W.finish_blocks > 0 ? W.finish_blocks : 4096), dim3(256), 0, st,
+                       (const FinScr *)fscr, cnt + C_KEPT, W.kept_cap, (uint64_t *)W.mem[W_SURV], cnt + C_SURV,
+                       W.surv_cap);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_fin_draw, dim3(4096), dim3(256), 0, st, (const uint64_t *)W.mem[W_SURV], cnt + C_SURV,
+                       W.surv_cap, (const FinDraw *)fdraw, fcnt);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_fin_rows, dim3(kgrid), dim3(256), 0, st, kept, cnt + C_KEPT, W.kept_cap, (const int2 *)fcnt,
+                       trk, ctrk, a.cn, a.r50, a.rt, a.cl, a.prefix, a.group, npts, n50_min, (PhxRow *)W.mem[W_ROWS],
+                       cnt + C_ROWS, W.row_cap);
     PHX_CK(hipGetLastError());
     if (n50_min > 0) {
         hipLaunchKernelGGL(k_rows_rtot, dim3(256), dim3(256), 0, st, trk, (const double *)T.mem[T_CTRK], a.cn, a.rt,
@@ -1091,8 +1158,10 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     mix(bits[0]); mix(bits[1]);
     const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt};
     for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
-    for (int i = 0; i < 16; i++) { mix((uint64_t)(uintptr_t)W.mem[i]); mix((uint64_t)(uintptr_t)T.mem[i]); }
+    for (int i = 0; i < 24; i++) mix((uint64_t)(uintptr_t)W.mem[i]);
+    for (int i = 0; i < 16; i++) mix((uint64_t)(uintptr_t)T.mem[i]);
     mix((uint64_t)W.cand_cap); mix((uint64_t)W.kept_cap); mix((uint64_t)W.row_cap); mix((uint64_t)W.item_cap);
+    mix((uint64_t)W.surv_cap);
     if (!W.gexec || W.gkey != key) {
         graph_drop(W);
         hipGraph_t g = nullptr;
@@ -1125,6 +1194,7 @@ int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
     if ((long)c[C_ITEMS] > W.item_cap) { W.item_cap = (long)c[C_ITEMS] + (long)c[C_ITEMS] / 4 + 4096; *grow = 1; }
     if ((long)c[C_KEPT] > W.kept_cap) { W.kept_cap = (long)c[C_KEPT] + (long)c[C_KEPT] / 4 + 4096; *grow = 1; }
     if ((long)c[C_ROWS] > W.row_cap) { W.row_cap = (long)c[C_ROWS] + (long)c[C_ROWS] / 4 + 4096; *grow = 1; }
+    if ((long)c[C_SURV] > W.surv_cap) { W.surv_cap = (long)c[C_SURV] + (long)c[C_SURV] / 4 + 4096; *grow = 1; }
     h[0] = (long)c[C_KEPT];
     h[1] = (long)c[C_ROWS];
     return 0;
@@ -1151,9 +1221,13 @@ void fast_release(FastTables &T, FastWork &W) {
     graph_drop(W);
     for (int i = 0; i < 16; i++) {
         if (T.mem[i]) (void)hipFree(T.mem[i]);
+        T.mem[i] = nullptr;
+        T.bytes[i] = 0;
+    }
+    for (int i = 0; i < 24; i++) {
         if (W.mem[i]) (void)hipFree(W.mem[i]);
-        T.mem[i] = W.mem[i] = nullptr;
-        T.bytes[i] = W.bytes[i] = 0;
+        W.mem[i] = nullptr;
+        W.bytes[i] = 0;
     }
 }
 

@@ -10,25 +10,29 @@
 // Record of an object tile (built from its FP64 track):
 //   a, b   least-squares line p'_k ~ a + b (k - tc), tc = 8j + 4, through the rotated positions
 //   e      largest distance of a rotated position from that line (the tube radius)
-//   U      3x3 matrix with U >= R_k S_k R_k^T for every point of the tile (Loewner order): the rotated
-//          covariance of the tile's last point plus eta I, eta >= the largest eigenvalue of
-//          R_k S_k R_k^T - R_kl S_kl R_kl^T over the tile; clusters add the pair's kernel R^2 I
-//   lam    upper bound of the largest eigenvalue of U (the smaller of trace and Gershgorin bound)
+//   U      3x3 matrix with U >= R_k S_k R_k^T for every point of the tile (Loewner order): either the
+//          rotated covariance of the tile's last point plus eta1 I, or the sum of the first and the last
+//          point's plus eta2 I, eta >= the largest eigenvalue of R_k S_k R_k^T minus that base over the tile
+//          (the one with the larger tr(U^-1)); clusters add the pair's kernel R^2 I
+//   lam    upper bound of the largest eigenvalue of U (the smaller of trace and Gershgorin bound, of the
+//          two choices)
 //
 // For a pair (star s, cluster c) and a point k of tile j, the reference's value is
 // chi2_k = min(q(m0), q(ms)) with q(x) = x^T M_k^-1 x, M_k = S_s,k + S_c,k + R^2 I, m0 = p_s,k - p_c,k and
-// ms a point of the segment m0 -> m1 = p_s,k+1 - p_c,k+1.  Rotated by R_k, m0 lies within e_s + e_c of the
-// axis da + db t (da = a_s - a_c, db = b_s - b_c, t in [-4, t1]); m1 rotated by R_k instead of R_k+1 moves
-// by at most ROT_W |m1|, and every point of the segment is a convex combination of its ends.  So every x
-// the reference evaluates in the tile lies in the tube  T = {da + db t + r : t in [-4, t1], |r| <= rho},
-// rho = e_s + e_c + ROT_W (|da| + 4 |db| + e_s + e_c), and q(x) >= x^T U^-1 x with U = U_s + U_c.
+// ms a point of the segment m0 -> m1 = p_s,k+1 - p_c,k+1.  Rotated by R_k, m0 lies in the tube
+// T = {da + db t + r : t in [-4, t1], |r| <= rho}, da = a_s - a_c, db = b_s - b_c, rho = e_s + e_c; m1 rotated
+// by R_k+1 lies in T too, so rotated by R_k it lies in R_1^-1 T (R_1 the rotation by OMEGA_PT), within
+// ROT_W |m1| of T.  Every point of the segment is a convex combination of its ends, and
+// q(x) >= x^T U^-1 x with U = U_s + U_c.
 //
-//   level 1 (FP32): x^T U^-1 x >= |x|^2 / lam(U) >= (dist(0, axis segment) - rho)^2 / (lam_s + lam_c)
-//   level 2 (FP64): for every vector y, x^T U^-1 x >= 2 y.x - y^T U y (the quadratic form's dual), so
-//                   min over T >= 2 (min_t y.(da + db t) - rho |y|) - y^T U y; y = U^-1 x* with x* the
-//                   U^-1-closest point of the axis segment makes the bound tight.  Evaluated as
-//                   det^2 times it with z = det y = adj(U) x*, and compared with an explicit bound of its
-//                   own rounding error.
+//   level 1 (FP32): x^T U^-1 x >= |x|^2 / lam(U) >= (dist(0, axis segment) - rho')^2 / (lam_s + lam_c),
+//                   rho' = rho + ROT_W (|da| + 4 |db| + rho) covering the rotated segment ends
+//   level 2 (FP64): for every vector y, x^T U^-1 x >= 2 y.x - y^T U y (the quadratic form's dual), and a
+//                   linear function's minimum over the hull of T and R_1^-1 T is its minimum over either,
+//                   so min >= 2 (min(min_t y.(da + db t), min_t (R_1 y).(da + db t)) - rho |y|) - y^T U y;
+//                   y = U^-1 x* with x* the U^-1-closest point of the axis segment makes the bound tight.
+//                   Evaluated as det^2 times it with z = det y = adj(U) x*, and compared with an explicit
+//                   bound of its own rounding error.
 //
 // A tile that fails either test holds no point with chi2 < chi2_max (1 + CUT_REL), so it can hold neither
 // the minimum of a kept pair nor change which pairs are kept.  CUT_REL covers the reference's own FP64
@@ -44,6 +48,8 @@ constexpr int TILE = 8;
 constexpr double OMEGA_PT = -0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122;   // frame rotation per 0.5-Myr point
 constexpr double ROT_W = 0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122 * 1.000001;   // |R(w) x - x| <= ROT_W |x|
 constexpr double CUT_REL = 1e-6;
+// R_1, the rotation by OMEGA_PT about z: cos and sin (to double precision)
+constexpr double ROT_C = 0.9999040755528862, ROT_S = -0.013850620662200556;
 
 struct TileRec1 {   // FP32, level 1
     float ax, ay, az, e, bx, by, bz, lam;
@@ -108,6 +114,13 @@ PHX_HDF void rot_cov(const double *S, double c, double s, double *o) {
     o[5] = S[5];
 }
 
+// tr(S^-1) of a symmetric 3x3 (xx xy xz yy yz zz); 0 when S is not positive definite.
+PHX_HDF double inv_trace(const double *S) {
+    const double c0 = S[3] * S[5] - S[4] * S[4], c3 = S[0] * S[5] - S[2] * S[2], c5 = S[0] * S[3] - S[1] * S[1];
+    const double det = S[0] * c0 + S[1] * (S[2] * S[4] - S[1] * S[5]) + S[2] * (S[1] * S[4] - S[2] * S[3]);
+    return det > 0.0 ? (c0 + c3 + c5) / det : 0.0;
+}
+
 // Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).
 PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2) {
     const int k0 = j * TILE;
@@ -153,13 +166,20 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
             e32 = fmax(e32, sqrt(fx * fx + fy * fy + fz * fz));
         }
     }
-    // Loewner bound of the rotated covariances of the tile's points
-    double ref[6];
+    // Loewner bounds of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
+    // and, for tiles of more than one point, sum + eta2 I (sum = the first point's + the last point's; over
+    // a tile the covariance grows nearly quadratically in time, so every point's lies close to or below the
+    // chord between the ends, and eta2 is usually 0 even where the ellipsoid shears and eta1 is as large as
+    // its narrow axes).  Level 2 takes the one with the larger tr(U^-1); level 1 the smaller eigenvalue bound.
+    double ref[6], sum[6];
     {
-        const double th = OMEGA_PT * kp;
+        const double th = OMEGA_PT * kp, t0 = OMEGA_PT * k0;
         rot_cov(trk + (size_t)kp * 12 + 6, cos(th), sin(th), ref);
+        rot_cov(trk + (size_t)k0 * 12 + 6, cos(t0), sin(t0), sum);
+#pragma unroll
+        for (int u = 0; u < 6; u++) sum[u] += ref[u];
     }
-    double eta = 0.0, trmax = ref[0] + ref[3] + ref[5];
+    double eta1 = 0.0, eta2 = 0.0, trmax = sum[0] + sum[3] + sum[5];
 #pragma unroll
     for (int i = 0; i < TILE; i++) {
         const int k = k0 + i;
@@ -167,19 +187,26 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
             const double th = OMEGA_PT * k;
             double q[6];
             rot_cov(trk + (size_t)k * 12 + 6, cos(th), sin(th), q);
-            double d[6];
+            double d1[6], d2[6];
 #pragma unroll
-            for (int u = 0; u < 6; u++) d[u] = q[u] - ref[u];
-            eta = fmax(eta, sym3_lmax_bound(d));
+            for (int u = 0; u < 6; u++) {
+                d1[u] = q[u] - ref[u];
+                d2[u] = q[u] - sum[u];
+            }
+            eta1 = fmax(eta1, sym3_lmax_bound(d1));
+            eta2 = fmax(eta2, sym3_lmax_bound(d2));
             trmax = fmax(trmax, q[0] + q[3] + q[5]);
         }
     }
-    eta = eta * (1.0 + 1e-12) + 1e-13 * trmax;   // rounding of the rotations and differences
-    double U[6] = {ref[0] + eta + R2, ref[1], ref[2], ref[3] + eta + R2, ref[4], ref[5] + eta + R2};
+    eta1 = eta1 * (1.0 + 1e-12) + 1e-13 * trmax;   // rounding of the rotations, sums and differences
+    eta2 = eta2 * (1.0 + 1e-12) + 1e-13 * trmax;
+    const double U1[6] = {ref[0] + eta1 + R2, ref[1], ref[2], ref[3] + eta1 + R2, ref[4], ref[5] + eta1 + R2};
+    const double U2[6] = {sum[0] + eta2 + R2, sum[1], sum[2], sum[3] + eta2 + R2, sum[4], sum[5] + eta2 + R2};
+    const double *U = kp > k0 && inv_trace(U2) > inv_trace(U1) ? U2 : U1;
     r1.ax = af[0]; r1.ay = af[1]; r1.az = af[2];
     r1.bx = bf[0]; r1.by = bf[1]; r1.bz = bf[2];
     r1.e = up(e32 * (1.0 + 1e-6) + 1e-12);
-    r1.lam = up(lam_bound(U) * (1.0 + 1e-6));
+    r1.lam = up(fmin(lam_bound(U1), lam_bound(U2)) * (1.0 + 1e-6));
 #pragma unroll
     for (int u = 0; u < 3; u++) { r2.a[u] = a[u]; r2.b[u] = b[u]; }
 #pragma unroll
@@ -244,7 +271,7 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     const double na = sqrt(da0 * da0 + da1 * da1 + da2 * da2) * (1.0 + 1e-12);
     const double nb = sqrt(db0 * db0 + db1 * db1 + db2 * db2) * (1.0 + 1e-12);
     const double e = s.e + c.e;
-    const double rho = (e + ROT_W * (na + 0.5 * TILE * nb + e)) * (1.0 + 1e-12);
+    const double rho = e * (1.0 + 1e-12);
     const double G0 = U[3] * U[5] - U[4] * U[4], G1 = U[2] * U[4] - U[1] * U[5], G2 = U[1] * U[4] - U[2] * U[3];
     const double G3 = U[0] * U[5] - U[2] * U[2], G4 = U[1] * U[2] - U[0] * U[4], G5 = U[0] * U[3] - U[1] * U[1];
     const double det = U[0] * G0 + U[1] * G1 + U[2] * G2;
@@ -259,7 +286,11 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     th = fmin(fmax(th, t0), t1);
     const double z0 = ga0 + th * gb0, z1 = ga1 + th * gb1, z2 = ga2 + th * gb2;
     const double za = z0 * da0 + z1 * da1 + z2 * da2, zb = z0 * db0 + z1 * db1 + z2 * db2;
-    const double lin = fmin(za + t0 * zb, za + t1 * zb);
+    // the segment ends m1 lie in R_1^-1 T (R_1: the rotation by OMEGA_PT), where z.x takes the values of
+    // (R_1 z).x' over T
+    const double zr0 = ROT_C * z0 - ROT_S * z1, zr1 = ROT_S * z0 + ROT_C * z1;
+    const double zra = zr0 * da0 + zr1 * da1 + z2 * da2, zrb = zr0 * db0 + zr1 * db1 + z2 * db2;
+    const double lin = fmin(fmin(za + t0 * zb, za + t1 * zb), fmin(zra + t0 * zrb, zra + t1 * zrb));
     const double nz2 = z0 * z0 + z1 * z1 + z2 * z2;
     const double nz = sqrt(nz2) * (1.0 + 1e-12);
     const double u0 = U[0] * z0 + U[1] * z1 + U[2] * z2, u1 = U[1] * z0 + U[3] * z1 + U[4] * z2,

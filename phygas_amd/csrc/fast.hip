@@ -15,8 +15,8 @@
 //   5. k_expand      one item per (candidate, tile left), contiguous per candidate
 //      k_items       one lane per item: the reference's FP64 evaluation of the tile's points (two
 //                    Mahalanobis forms per point, strict-< first-index minimum)
-//      k_reduce      one lane per candidate: minimum over its items; pairs with chi2 < chi2_max are appended
-//                    to the kept list
+//      k_reduce      eight lanes per candidate: minimum over its items; pairs with chi2 < chi2_max are
+//                    appended to the kept list
 //   6. k_finish_fast one wavefront per 8 kept pairs: the 512 Box-Muller draws of each pair's seed, screened
 //                    draw by draw (below) with the rare survivors evaluated through a wave-local queue;
 //                    exact counts; rows with p_r50 >= p_min appended
@@ -508,50 +508,65 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
     }
 }
 
-// One lane per candidate: minimum over its items in tile order (the items carry the segment parameter of
-// their minimum); kept pairs are appended to the kept list (one atomic per wavefront).
+// Eight lanes per candidate: lane p takes its items p, p + 8, ...; the lexicographic (chi2, k) minimum over
+// the lanes is the reference's strict-< first-index minimum (items are in tile order and carry the segment
+// parameter of their minimum).  Kept pairs wait in a per-wavefront LDS ring and go to the kept list 64 at a
+// time (one atomic per 64 keeps the counter uncontended).
+constexpr int KQ = 128;
 __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
                                                 long cand_cap, const long *__restrict__ istart,
                                                 const IRes *__restrict__ res, const unsigned long long *__restrict__ nitems,
                                                 long item_cap, double chi2_max, Kept *__restrict__ kept,
                                                 unsigned long long *__restrict__ nkept, long kept_cap) {
+    __shared__ Kept ring_s[4][KQ];
     const long n = min((long)*ncand, cand_cap);
     if ((long)*nitems > item_cap) return;   // the pass is run again with a larger item list
-    const int lane = threadIdx.x & 63;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long b0 = (long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b0 < n; b0 += stride) {   // wave-uniform
-        const long id = b0 + lane;
-        Kept kp = {};
-        bool hit = false;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, p = lane & 7, g = lane >> 3;
+    Kept *ring = ring_s[w];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int head = 0, tail = 0;   // wave-uniform
+    auto flush = [&]() {
+        const int nf = min(tail - head, 64);
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(nkept, (unsigned long long)nf);
+        base = __shfl(base, 0, 64);
+        if (lane < nf && (long)(base + lane) < kept_cap) kept[base + lane] = ring[(head + lane) & (KQ - 1)];
+        head += nf;
+        wave_sync();
+    };
+    const long stride = ((long)gridDim.x * blockDim.x) >> 3;
+    for (long b0 = ((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> 3; b0 < n; b0 += stride) {   // wave-uniform
+        const long id = b0 + g;
+        double best = __builtin_inf(), sb = 0.0;
+        int kb = 0x7fffffff;
+        Cand cd = {0, 0, 0ull};
         if (id < n) {
-            const Cand cd = cand[id];
+            cd = cand[id];
             const long i0 = istart[id];
             const int ni = __popcll(cd.mask);
-            double best = __builtin_inf(), sb = 0.0;
-            int kb = 0;
-            for (int i = 0; i < ni; i += 4) {   // four results in flight
-                IRes r[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) r[u] = i + u < ni ? res[i0 + i + u] : IRes{__builtin_inf(), 0.0, 0, 0};
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (r[u].q < best) { best = r[u].q; kb = r[u].k; sb = r[u].sv; }
-            }
-            if (best < chi2_max) {
-                kp = Kept{cd.star, cd.cluster, kb, 0, sb, best};
-                hit = true;
+            for (int i = p; i < ni; i += 8) {
+                const IRes q = res[i0 + i];
+                if (q.q < best || (q.q == best && q.k < kb)) { best = q.q; kb = q.k; sb = q.sv; }
             }
         }
-        const unsigned long long b = __ballot(hit);
-        if (b != 0ull) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(nkept, (unsigned long long)__popcll(b));
-            base = __shfl(base, 0, 64);
-            if (hit) {
-                const unsigned long long o = base + __popcll(b & ((1ull << lane) - 1ull));
-                if ((long)o < kept_cap) kept[o] = kp;
-            }
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+            const double qo = __shfl_xor(best, d, 64), so = __shfl_xor(sb, d, 64);
+            const int ko = __shfl_xor(kb, d, 64);
+            if (qo < best || (qo == best && ko < kb)) { best = qo; kb = ko; sb = so; }
         }
+        const bool hit = id < n && p == 0 && best < chi2_max;
+        const unsigned long long bal = __ballot(hit);
+        if (hit) ring[(tail + __popcll(bal & below)) & (KQ - 1)] = Kept{cd.star, cd.cluster, kb, 0, sb, best};
+        tail += __popcll(bal);
+        if (tail - head >= 64) {   // wave-uniform
+            wave_sync();
+            flush();
+        }
+    }
+    if (tail > head) {   // wave-uniform
+        wave_sync();
+        flush();
     }
 }
 

@@ -34,3 +34,8 @@ extern "C" int cc_scan(const double *strk, int ns, int snpts, const double *ctrk
     delete[] s2;
     return ntl;
 }
+
+// Largest difference of the level-2 rotation constants from cos / sin of the frame rotation per point.
+extern "C" double cc_rot_check(void) {
+    return fmax(fabs(ROT_C - cos(OMEGA_PT)), fabs(ROT_S - sin(OMEGA_PT)));
+}

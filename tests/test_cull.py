@@ -107,3 +107,9 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     f1 = p1[np.broadcast_to(exists, p1.shape)].mean()
     f2 = p2[np.broadcast_to(exists, p2.shape)].mean()
     assert f1 < 0.35 and f2 < 0.12, (f1, f2)
+
+
+def test_level2_rotation_constants(harness):
+    """The level-2 test rotates its dual vector by one point's frame rotation with literal cos / sin."""
+    harness.cc_rot_check.restype = C.c_double
+    assert harness.cc_rot_check() < 1e-15

@@ -30,9 +30,8 @@
 //   level 2 (FP64): for every vector y, x^T U^-1 x >= 2 y.x - y^T U y (the quadratic form's dual), and a
 //                   linear function's minimum over the hull of T and R_1^-1 T is its minimum over either,
 //                   so min >= 2 (min(min_t y.(da + db t), min_t (R_1 y).(da + db t)) - rho |y|) - y^T U y;
-//                   y = U^-1 x* with x* the U^-1-closest point of the axis segment makes the bound tight.
-//                   Evaluated as det^2 times it with z = det y = adj(U) x*, and compared with an explicit
-//                   bound of its own rounding error.
+//                   y along U^-1 x* with x* the U^-1-closest point of the axis segment, scaled optimally,
+//                   makes the bound tight (l2_pass), compared with explicit bounds of its rounding.
 //
 // A tile that fails either test holds no point with chi2 < chi2_max (1 + CUT_REL), so it can hold neither
 // the minimum of a kept pair nor change which pairs are kept.  CUT_REL covers the reference's own FP64
@@ -257,7 +256,12 @@ PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi)
     return lhs * 0.99996f <= rhs;
 }
 
-// Level 2, FP64.  chi2c = chi2_max (1 + CUT_REL).
+// Level 2, FP64.  chi2c = chi2_max (1 + CUT_REL).  For y = s z (any z, s > 0) the dual bound is
+// 2 s L - s^2 z^T U z with L = min over the hull of z.x = min(lin(z), lin(R_1 z)) - rho |z|, largest at
+// s = L / z^T U z: every point has q >= L^2 / z^T U z when L > 0.  z = adj(U) x* (x* the U^-1-closest point
+// of the axis segment) up to a positive factor, which needs no division; any z is valid, so only the
+// rounding of L and z^T U z is bounded (err_l, err_u).  U is positive definite: its diagonal holds the
+// kernel term R^2 > 0 and the trace guard bounds its condition number.
 PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi2c) {
     const double da0 = s.a[0] - c.a[0], da1 = s.a[1] - c.a[1], da2 = s.a[2] - c.a[2];
     const double db0 = s.b[0] - c.b[0], db1 = s.b[1] - c.b[1], db2 = s.b[2] - c.b[2];
@@ -268,23 +272,20 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     if (!(tr < 1e9 * (s.R2 + c.R2))) return true;   // condition number not bounded: never pruned here
     const double dlt = 1e-13 * tr;                    // U rounded up over the rounding of the sum
     U[0] += dlt; U[3] += dlt; U[5] += dlt;
-    const double na = sqrt(da0 * da0 + da1 * da1 + da2 * da2) * (1.0 + 1e-12);
-    const double nb = sqrt(db0 * db0 + db1 * db1 + db2 * db2) * (1.0 + 1e-12);
-    const double e = s.e + c.e;
-    const double rho = e * (1.0 + 1e-12);
+    const double rho = (s.e + c.e) * (1.0 + 1e-12);
     const double G0 = U[3] * U[5] - U[4] * U[4], G1 = U[2] * U[4] - U[1] * U[5], G2 = U[1] * U[4] - U[2] * U[3];
     const double G3 = U[0] * U[5] - U[2] * U[2], G4 = U[1] * U[2] - U[0] * U[4], G5 = U[0] * U[3] - U[1] * U[1];
-    const double det = U[0] * G0 + U[1] * G1 + U[2] * G2;
-    if (!(det > 0.0)) return true;
     const double ga0 = G0 * da0 + G1 * da1 + G2 * da2, ga1 = G1 * da0 + G3 * da1 + G4 * da2,
                  ga2 = G2 * da0 + G4 * da1 + G5 * da2;
     const double gb0 = G0 * db0 + G1 * db1 + G2 * db2, gb1 = G1 * db0 + G3 * db1 + G4 * db2,
                  gb2 = G2 * db0 + G4 * db1 + G5 * db2;
     const double B = da0 * gb0 + da1 * gb1 + da2 * gb2, C = db0 * gb0 + db1 * gb1 + db2 * gb2;
     const double t0 = -0.5 * TILE;
-    double th = C > 0.0 ? -B / C : t0;
-    th = fmin(fmax(th, t0), t1);
-    const double z0 = ga0 + th * gb0, z1 = ga1 + th * gb1, z2 = ga2 + th * gb2;
+    // z = C (ga + th gb) with th = -B / C inside [t0, t1], else ga + t gb at the nearer end
+    double wa = C, wb = -B;
+    if (!(C > 0.0) || -B <= t0 * C) { wa = 1.0; wb = t0; }
+    else if (-B >= t1 * C) { wa = 1.0; wb = t1; }
+    const double z0 = wa * ga0 + wb * gb0, z1 = wa * ga1 + wb * gb1, z2 = wa * ga2 + wb * gb2;
     const double za = z0 * da0 + z1 * da1 + z2 * da2, zb = z0 * db0 + z1 * db1 + z2 * db2;
     // the segment ends m1 lie in R_1^-1 T (R_1: the rotation by OMEGA_PT), where z.x takes the values of
     // (R_1 z).x' over T
@@ -296,10 +297,11 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     const double u0 = U[0] * z0 + U[1] * z1 + U[2] * z2, u1 = U[1] * z0 + U[3] * z1 + U[4] * z2,
                  u2 = U[2] * z0 + U[4] * z1 + U[5] * z2;
     const double zuz = z0 * u0 + z1 * u1 + z2 * u2;
-    const double thr = chi2c * det * det;
-    const double lbd = 2.0 * det * (lin - rho * nz) - zuz;   // det^2 x (the lower bound)
-    const double err = 1e-12 * (2.0 * det * (nz * (na + 0.5 * TILE * nb) + rho * nz) + tr * nz2 + thr);
-    return !(lbd - thr > err);
+    const double na1 = fabs(da0) + fabs(da1) + fabs(da2), nb1 = fabs(db0) + fabs(db1) + fabs(db2);
+    const double err_l = 1e-12 * nz * (na1 + 0.5 * TILE * nb1 + rho);
+    const double err_u = 1e-12 * tr * nz2;
+    const double L = lin - rho * nz - err_l;
+    return !(L > 0.0 && L * L > chi2c * (zuz + err_u) * (1.0 + 1e-12));
 }
 
 }  // namespace phx

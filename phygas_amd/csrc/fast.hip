@@ -9,7 +9,7 @@
 //   3. k_star_recs   per (star, 8-point tile): the tile records of cull.h (axis line, tube radius and
 //                    Loewner covariance bound in the co-rotating frame)
 //   4. k_scan_tiles  persistent, length-bucketed: each wavefront takes 64 clusters (sorted by track length)
-//                    x 32 stars, tests every (pair, tile) with the FP32 tube test and the survivors with the
+//                    x 16 stars, tests every (pair, tile) with the FP32 tube test and the survivors with the
 //                    FP64 Mahalanobis certificate (cull.h), and appends the pairs with any tile left, with a
 //                    bit mask of those tiles
 //   5. k_expand      one item per (candidate, tile left), contiguous per candidate
@@ -49,7 +49,7 @@
 namespace phx {
 
 // ------------------------------------------------------------------------------------------- layout
-constexpr int SPW = 32;                 // stars per wavefront in the tile scan
+constexpr int SPW = 16;                 // stars per wavefront in the tile scan
 
 struct Cand { int star, cluster; unsigned long long mask; };
 struct Kept { int star, cluster, kb, pad; double sb, chi2; };

@@ -113,6 +113,46 @@ PHX_HDF void rot_cov(const double *S, double c, double s, double *o) {
     o[5] = S[5];
 }
 
+// Positive part of a symmetric 3x3 (xx xy xz yy yz zz): V diag(max(lambda, 0)) V^T from cyclic Jacobi
+// rotations.  Approximate (the callers bound what it misses).
+PHX_HDF void sym3_pos_part(const double *D, double *P) {
+    double a[3][3] = {{D[0], D[1], D[2]}, {D[1], D[3], D[4]}, {D[2], D[4], D[5]}};
+    double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const double scale = fabs(D[0]) + fabs(D[3]) + fabs(D[5]) + fabs(D[1]) + fabs(D[2]) + fabs(D[4]);
+    for (int sweep = 0; sweep < 6; sweep++) {
+        for (int r = 0; r < 3; r++) {
+            const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2;
+            const double apq = a[p][q];
+            if (!(fabs(apq) > 1e-18 * scale)) continue;
+            const double tau = (a[q][q] - a[p][p]) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+            for (int k = 0; k < 3; k++) {   // A <- A J (columns p, q)
+                const double akp = a[k][p], akq = a[k][q];
+                a[k][p] = c * akp - sn * akq;
+                a[k][q] = sn * akp + c * akq;
+            }
+            for (int k = 0; k < 3; k++) {   // A <- J^T A (rows p, q)
+                const double apk = a[p][k], aqk = a[q][k];
+                a[p][k] = c * apk - sn * aqk;
+                a[q][k] = sn * apk + c * aqk;
+            }
+            for (int k = 0; k < 3; k++) {   // V <- V J
+                const double vkp = v[k][p], vkq = v[k][q];
+                v[k][p] = c * vkp - sn * vkq;
+                v[k][q] = sn * vkp + c * vkq;
+            }
+        }
+    }
+    const double l[3] = {fmax(a[0][0], 0.0), fmax(a[1][1], 0.0), fmax(a[2][2], 0.0)};
+    P[0] = l[0] * v[0][0] * v[0][0] + l[1] * v[0][1] * v[0][1] + l[2] * v[0][2] * v[0][2];
+    P[1] = l[0] * v[0][0] * v[1][0] + l[1] * v[0][1] * v[1][1] + l[2] * v[0][2] * v[1][2];
+    P[2] = l[0] * v[0][0] * v[2][0] + l[1] * v[0][1] * v[2][1] + l[2] * v[0][2] * v[2][2];
+    P[3] = l[0] * v[1][0] * v[1][0] + l[1] * v[1][1] * v[1][1] + l[2] * v[1][2] * v[1][2];
+    P[4] = l[0] * v[1][0] * v[2][0] + l[1] * v[1][1] * v[2][1] + l[2] * v[1][2] * v[2][2];
+    P[5] = l[0] * v[2][0] * v[2][0] + l[1] * v[2][1] * v[2][1] + l[2] * v[2][2] * v[2][2];
+}
+
 // tr(S^-1) of a symmetric 3x3 (xx xy xz yy yz zz); 0 when S is not positive definite.
 PHX_HDF double inv_trace(const double *S) {
     const double c0 = S[3] * S[5] - S[4] * S[4], c3 = S[0] * S[5] - S[2] * S[2], c5 = S[0] * S[3] - S[1] * S[1];
@@ -174,9 +214,13 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     {
         const double th = OMEGA_PT * kp, t0 = OMEGA_PT * k0;
         rot_cov(trk + (size_t)kp * 12 + 6, cos(th), sin(th), ref);
-        rot_cov(trk + (size_t)k0 * 12 + 6, cos(t0), sin(t0), sum);
+        double first[6], d[6], pp[6];
+        rot_cov(trk + (size_t)k0 * 12 + 6, cos(t0), sin(t0), first);
 #pragma unroll
-        for (int u = 0; u < 6; u++) sum[u] += ref[u];
+        for (int u = 0; u < 6; u++) d[u] = first[u] - ref[u];
+        sym3_pos_part(d, pp);
+#pragma unroll
+        for (int u = 0; u < 6; u++) sum[u] = ref[u] + pp[u];
     }
     double eta1 = 0.0, eta2 = 0.0, trmax = sum[0] + sum[3] + sum[5];
 #pragma unroll

@@ -29,13 +29,18 @@ PHX_HDF float f_sqrt(float x) { return __builtin_sqrtf(x); }
 PHX_HDF float f_log2(float x) { return __builtin_amdgcn_logf(x); }
 PHX_HDF float f_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 PHX_HDF float f_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+PHX_HDF float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 #else
 PHX_HDF float f_rsqrt(float x) { return 1.0f / sqrtf(x); }
 PHX_HDF float f_sqrt(float x) { return sqrtf(x); }
 PHX_HDF float f_log2(float x) { return log2f(x); }
 PHX_HDF float f_exp2(float x) { return exp2f(x); }
 PHX_HDF float f_fma(float a, float b, float c) { return fmaf(a, b, c); }
+PHX_HDF float f_rcp(float x) { return 1.0f / x; }
 #endif
+// a / b through the hardware reciprocal (within ~2 ulp; a reciprocal shared by several quotients is
+// evaluated once)
+PHX_HDF float f_div(float a, float b) { return a * f_rcp(b); }
 
 struct PotF {
     float bulge_gm, b2, disk_gm, disk_a, db2, halo_k1, halo_k2, halo_cut2, halo_gmcut, halo_p1, halo_p2;
@@ -110,34 +115,34 @@ PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy
     // bulge: -GM p u^-3/2, u = r^2 + b^2
     const float u0 = r2 + Q.b2;
     const float iu0 = f_rsqrt(u0), g0 = iu0 * iu0 * iu0;
-    const float dg = g0 * pw1(dr2 / u0, -1.5f);
+    const float dg = g0 * pw1(f_div(dr2, u0), -1.5f);
     const float g1 = g0 + dg;
     // disk: -GM (x, y, z (a + zeta) / zeta) D^-3, D^2 = R^2 + (a + zeta)^2, zeta = sqrt(z^2 + b^2)
     const float zeta0 = f_sqrt(f_fma(z, z, Q.db2)), zeta1 = f_sqrt(f_fma(z + dz, z + dz, Q.db2));
-    const float dzeta = dz * (2.0f * z + dz) / (zeta0 + zeta1);
+    const float dzeta = f_div(dz * (2.0f * z + dz), zeta0 + zeta1);
     const float A0 = Q.disk_a + zeta0;
     const float D20 = f_fma(A0, A0, R2);
     const float dD2 = f_fma(dzeta, 2.0f * A0 + dzeta, dR2);
     const float iD0 = f_rsqrt(D20), h0 = iD0 * iD0 * iD0;
-    const float dh = h0 * pw1(dD2 / D20, -1.5f);
+    const float dh = h0 * pw1(f_div(dD2, D20), -1.5f);
     const float h1 = h0 + dh;
     // z (1 + a / zeta): difference of z / zeta is (dz zeta0 - z dzeta) / (zeta0 zeta1)
-    const float q0 = z * (1.0f + Q.disk_a / zeta0);
-    const float dq = dz + Q.disk_a * (dz * zeta0 - z * dzeta) / (zeta0 * zeta1);
+    const float q0 = z * (1.0f + f_div(Q.disk_a, zeta0));
+    const float dq = dz + f_div(Q.disk_a * (dz * zeta0 - z * dzeta), zeta0 * zeta1);
     // halo
     float f0, df;
     const float r21 = r2 + dr2;
     if (r2 < Q.halo_cut2 && r21 < Q.halo_cut2) {
-        const float t = dr2 / r2;
+        const float t = f_div(dr2, r2);
         const float N0 = f_pow(r2, Q.halo_p1), E0 = Q.halo_k2 * f_pow(r2, Q.halo_p2);
         const float dN = N0 * pw1(t, Q.halo_p1), dE = E0 * pw1(t, Q.halo_p2);
         const float Dn0 = 1.0f + E0, Dn1 = Dn0 + dE;
-        f0 = Q.halo_k1 * N0 / Dn0;
-        df = Q.halo_k1 * (dN * Dn0 - N0 * dE) / (Dn0 * Dn1);
+        f0 = f_div(Q.halo_k1 * N0, Dn0);
+        df = f_div(Q.halo_k1 * (dN * Dn0 - N0 * dE), Dn0 * Dn1);
     } else if (r2 >= Q.halo_cut2 && r21 >= Q.halo_cut2) {
         const float ir = f_rsqrt(r2);
         f0 = Q.halo_gmcut * ir * ir * ir;
-        df = f0 * pw1(dr2 / r2, -1.5f);
+        df = f0 * pw1(f_div(dr2, r2), -1.5f);
     } else {   // straddles the cutoff: plain difference
         const float ir0 = f_rsqrt(r2), ir1 = f_rsqrt(r21);
         f0 = r2 < Q.halo_cut2 ? Q.halo_k1 * f_pow(r2, Q.halo_p1) / (1.0f + Q.halo_k2 * f_pow(r2, Q.halo_p2))

@@ -37,6 +37,7 @@
 // The 1e-3 bound on |z estimate - z| holds with a factor ~40 to spare (phx_check_finish_screen measures it,
 // tail draws included).
 #include <math.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -740,6 +741,24 @@ __device__ int fin_draw(const FinDraw &f, int i) {
     return (rr < f.r ? 1 : 0) | (rr < f.ra ? 2 : 0);
 }
 
+// fnv_feed(h, name, 40) of a cluster name (the first field of PhCluster, so 8-byte aligned): the 40 bytes
+// through five 8-byte loads instead of a chain of byte loads, the bytes after the first NUL left out.
+static_assert(offsetof(PhCluster, name) == 0 && sizeof(PhCluster) % 8 == 0, "cluster names are 8-byte aligned");
+__device__ __forceinline__ uint64_t fnv_name40(uint64_t h, const char *__restrict__ name) {
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(name);
+    uint64_t v[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) v[i] = w[i];
+    bool live = true;
+#pragma unroll
+    for (int i = 0; i < 40; i++) {
+        const uint64_t ch = (v[i >> 3] >> (8 * (i & 7))) & 0xffull;   // little-endian byte i
+        live = live && ch != 0;
+        h = live ? (h ^ ch) * FNV_PRIME : h;
+    }
+    return h;
+}
+
 // Frame of the draws of a kept pair (star s, cluster c, point kb, segment parameter sb): the mean m (the
 // point of the step's segment), the covariance C = S_s + S_c at kb and its Cholesky factor L
 // (l00 l10 l20 l11 l21 l22), the seed of the pair's generator.  One function for every kernel that needs
@@ -762,7 +781,7 @@ __device__ __forceinline__ void pair_frame(const double *__restrict__ strk, cons
     L[3] = sqrt(fmax(C[3] - L[1] * L[1], 0.0));
     L[4] = L[3] > 0 ? (C[4] - L[2] * L[1]) / L[3] : 0;
     L[5] = sqrt(fmax(C[5] - L[2] * L[2] - L[4] * L[4], 0.0));
-    seed = fnv_feed(sprefix[s], cl[c].name, 40);
+    seed = fnv_name40(sprefix[s], cl[c].name);
 }
 
 // The finish runs as four kernels so that the draw loop, which is most of its work, keeps few registers

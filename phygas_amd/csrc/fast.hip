@@ -57,10 +57,14 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
-       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_NSLOT };
+       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_NSLOT };
 // Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
 // for the exact evaluation by the finish screen
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_N };
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_N };
+// The reduce appends kept pairs to KSEG segments of the W_KSEG list, each with its own counter (counter i at
+// C_KSEG0 + i KSEG_STRIDE, one per 256 bytes): one counter for every wavefront of the grid would take its
+// atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
+constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32, C_TOTAL = C_KSEG0 + KSEG * KSEG_STRIDE;
 // Table buffers (FastTables::mem slots)
 enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_NSLOT };
 
@@ -423,8 +427,10 @@ struct Item { int cand, star, cluster, j; };
 struct IRes { double q, sv; int k, pad; };
 
 // Items of the candidates, one per tile left by the scan, contiguous per candidate in increasing tile
-// order: istart[c] = first item of candidate c.  Grid-stride over blocks of 256 candidates; each block takes
-// its range of the item list with one atomic.
+// order: istart[c] = first item of candidate c.  Grid-stride over blocks of 256 x EXP_PER candidates (thread t
+// takes EXP_PER consecutive ones, so the item list keeps the candidates' order); each block takes its range of the item list with one atomic (few
+// enough that the counter is not contended).
+constexpr int EXP_PER = 8;
 __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
                                                 long cand_cap, Item *__restrict__ items,
                                                 unsigned long long *__restrict__ nitems, long item_cap,
@@ -433,10 +439,15 @@ __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, c
     __shared__ unsigned long long base_s;
     const long n = min((long)*ncand, cand_cap);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (long b0 = (long)blockIdx.x * blockDim.x; b0 < n; b0 += (long)gridDim.x * blockDim.x) {
-        const long id = b0 + threadIdx.x;
-        const Cand cd = id < n ? cand[id] : Cand{0, 0, 0ull};
-        const unsigned cnt = (unsigned)__popcll(cd.mask);
+    for (long b0 = (long)blockIdx.x * blockDim.x * EXP_PER; b0 < n; b0 += (long)gridDim.x * blockDim.x * EXP_PER) {
+        Cand cd[EXP_PER];
+        unsigned cnt = 0;
+#pragma unroll
+        for (int u = 0; u < EXP_PER; u++) {
+            const long id = b0 + threadIdx.x * EXP_PER + u;
+            cd[u] = id < n ? cand[id] : Cand{0, 0, 0ull};
+            cnt += (unsigned)__popcll(cd[u].mask);
+        }
         unsigned x = cnt;   // inclusive scan within the wavefront
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -453,15 +464,19 @@ __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, c
         }
         if (threadIdx.x == 0) base_s = atomicAdd(nitems, (unsigned long long)tot);
         __syncthreads();
-        const unsigned long long first = base_s + off + x - cnt;
-        if (id < n) {
-            istart[id] = (long)first;
-            unsigned long long mm = cd.mask, o = first;
-            while (mm) {
-                const int j = __builtin_ctzll(mm);
-                mm &= mm - 1ull;
-                if ((long)o < item_cap) items[o] = Item{(int)id, cd.star, cd.cluster, j};
-                o++;
+        unsigned long long o = base_s + off + x - cnt;
+#pragma unroll
+        for (int u = 0; u < EXP_PER; u++) {
+            const long id = b0 + threadIdx.x * EXP_PER + u;
+            if (id < n) {
+                istart[id] = (long)o;
+                unsigned long long mm = cd[u].mask;
+                while (mm) {
+                    const int j = __builtin_ctzll(mm);
+                    mm &= mm - 1ull;
+                    if ((long)o < item_cap) items[o] = Item{(int)id, cd[u].star, cd[u].cluster, j};
+                    o++;
+                }
             }
         }
         __syncthreads();   // wsum / base_s reused by the next block of candidates
@@ -511,8 +526,8 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
 
 // Eight lanes per candidate: lane p takes its items p, p + 8, ...; the lexicographic (chi2, k) minimum over
 // the lanes is the reference's strict-< first-index minimum (items are in tile order and carry the segment
-// parameter of their minimum).  Kept pairs wait in a per-wavefront LDS ring and go to the kept list 64 at a
-// time (one atomic per 64 keeps the counter uncontended).
+// parameter of their minimum).  Kept pairs wait in a per-wavefront LDS ring and go to the wavefront's
+// segment of the kept list 64 at a time (kept: KSEG segments of kept_cap / KSEG; nkept: their counters).
 constexpr int KQ = 128;
 __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
                                                 long cand_cap, const long *__restrict__ istart,
@@ -526,12 +541,15 @@ __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, c
     Kept *ring = ring_s[w];
     const unsigned long long below = (1ull << lane) - 1ull;
     int head = 0, tail = 0;   // wave-uniform
+    const int seg = (int)((((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) % KSEG);
+    const long seg_cap = kept_cap / KSEG;
+    Kept *kseg = kept + seg * seg_cap;
     auto flush = [&]() {
         const int nf = min(tail - head, 64);
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(nkept, (unsigned long long)nf);
+        if (lane == 0) base = atomicAdd(nkept + seg * KSEG_STRIDE, (unsigned long long)nf);
         base = __shfl(base, 0, 64);
-        if (lane < nf && (long)(base + lane) < kept_cap) kept[base + lane] = ring[(head + lane) & (KQ - 1)];
+        if (lane < nf && (long)(base + lane) < seg_cap) kseg[base + lane] = ring[(head + lane) & (KQ - 1)];
         head += nf;
         wave_sync();
     };
@@ -568,6 +586,43 @@ __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, c
     if (tail > head) {   // wave-uniform
         wave_sync();
         flush();
+    }
+}
+
+// Packs the KSEG segments of the reduce's list into the kept list; nkept[C_KEPT] = the total and
+// nkept[C_KSEGMAX] = the largest segment count (a segment over kept_cap / KSEG lost pairs: the pass is run
+// again with a larger list).
+__global__ __launch_bounds__(256) void k_kept_compact(const Kept *__restrict__ kseg, long kept_cap,
+                                                      unsigned long long *__restrict__ cnt, Kept *__restrict__ kept) {
+    __shared__ long pre_s[KSEG + 1];
+    __shared__ unsigned long long c_s[KSEG];
+    const long seg_cap = kept_cap / KSEG;
+    if (threadIdx.x < KSEG) c_s[threadIdx.x] = cnt[C_KSEG0 + threadIdx.x * KSEG_STRIDE];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long acc = 0;
+        unsigned long long mx = 0;
+        for (int i = 0; i < KSEG; i++) {
+            const unsigned long long c = c_s[i];
+            mx = c > mx ? c : mx;
+            pre_s[i] = acc;
+            acc += (long)min((long)c, seg_cap);
+        }
+        pre_s[KSEG] = acc;
+        if (blockIdx.x == 0) {
+            cnt[C_KEPT] = (unsigned long long)acc;
+            cnt[C_KSEGMAX] = mx;
+        }
+    }
+    __syncthreads();
+    const long total = pre_s[KSEG];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = KSEG;   // segment i with pre_s[i] <= e < pre_s[i + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre_s[mid] <= e) lo = mid; else hi = mid;
+        }
+        kept[e] = kseg[lo * seg_cap + (e - pre_s[lo])];
     }
 }
 
@@ -1000,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_z0_check(uint64_t seed, long n, unsigne
 }
 
 __global__ void k_zero_counters(unsigned long long *c, int n) {
-    if (threadIdx.x < n) c[threadIdx.x] = 0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0ull;
 }
 
 
@@ -1082,7 +1137,8 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
         ensure(W.mem[W_FCNT], W.bytes[W_FCNT], sizeof(int2) * (size_t)W.kept_cap) ||
         ensure(W.mem[W_SURV], W.bytes[W_SURV], sizeof(uint64_t) * (size_t)W.surv_cap) ||
 
-        ensure(W.mem[W_CNT], W.bytes[W_CNT], C_N * sizeof(unsigned long long)))
+        ensure(W.mem[W_KSEG], W.bytes[W_KSEG], sizeof(Kept) * (size_t)W.kept_cap) ||
+        ensure(W.mem[W_CNT], W.bytes[W_CNT], C_TOTAL * sizeof(unsigned long long)))
         return 1;
     return 0;
 }
@@ -1091,7 +1147,7 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
 static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
     const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     unsigned long long *cnt = (unsigned long long *)W.mem[W_CNT];
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cnt, C_N);
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, st, cnt, C_TOTAL);
     PHX_CK(hipGetLastError());
     double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
     TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
@@ -1136,7 +1192,10 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     if (mark) mark(mark_ctx, "items");
     hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 8192), dim3(256), 0, st, cand,
                        cnt + C_CAND, W.cand_cap, (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES],
-                       cnt + C_ITEMS, W.item_cap, a.chi2_max, (Kept *)W.mem[W_KEPT], cnt + C_KEPT, W.kept_cap);
+                       cnt + C_ITEMS, W.item_cap, a.chi2_max, (Kept *)W.mem[W_KSEG], cnt + C_KSEG0, W.kept_cap);
+    PHX_CK(hipGetLastError());
+    hipLaunchKernelGGL(k_kept_compact, dim3(1024), dim3(256), 0, st, (const Kept *)W.mem[W_KSEG], W.kept_cap, cnt,
+                       (Kept *)W.mem[W_KEPT]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "reduce");
     const int n50_min = a.p_min > 0 ? (int)ceil(a.p_min * 512 - 1e-9) : 0;
@@ -1226,7 +1285,10 @@ int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
     *grow = 0;
     if ((long)c[C_CAND] > W.cand_cap) { W.cand_cap = (long)c[C_CAND] + (long)c[C_CAND] / 4 + 4096; *grow = 1; }
     if ((long)c[C_ITEMS] > W.item_cap) { W.item_cap = (long)c[C_ITEMS] + (long)c[C_ITEMS] / 4 + 4096; *grow = 1; }
-    if ((long)c[C_KEPT] > W.kept_cap) { W.kept_cap = (long)c[C_KEPT] + (long)c[C_KEPT] / 4 + 4096; *grow = 1; }
+    if ((long)c[C_KSEGMAX] > W.kept_cap / KSEG) {   // a segment of the reduce's list overflowed
+        W.kept_cap = std::max(W.kept_cap, ((long)c[C_KSEGMAX] + (long)c[C_KSEGMAX] / 4 + 64) * KSEG);
+        *grow = 1;
+    }
     if ((long)c[C_ROWS] > W.row_cap) { W.row_cap = (long)c[C_ROWS] + (long)c[C_ROWS] / 4 + 4096; *grow = 1; }
     if ((long)c[C_SURV] > W.surv_cap) { W.surv_cap = (long)c[C_SURV] + (long)c[C_SURV] / 4 + 4096; *grow = 1; }
     h[0] = (long)c[C_KEPT];

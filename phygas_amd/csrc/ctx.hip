@@ -367,7 +367,8 @@ static int last_counts(PhxCtx *c, long *kept, long *nrows) {
         *kept = *nrows = 0;
         return 0;
     }
-    for (int attempt = 0; attempt < 4; attempt++) {
+    // each rerun can find one more overflowing stage (candidates, items, kept segments, listed draws, rows)
+    for (int attempt = 0; attempt < 8; attempt++) {
         long h[2];
         if (c->last_form == PHX_FORM_PLAIN) {
             PHX_CK(hipMemcpyAsync(h, c->counts.p, sizeof h, hipMemcpyDeviceToHost, c->st));

@@ -267,11 +267,7 @@ PHX_HDF float tile_t1(int n, int j) {
 }
 
 // Square root to within 1 ulp (the hardware instruction on the device).
-#if defined(__HIP_DEVICE_COMPILE__)
-PHX_HDF float f_sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
-#else
-PHX_HDF float f_sqrt_fast(float x) { return sqrtf(x); }
-#endif
+PHX_HDF float f_sqrt_fast(float x) { return f_sqrt_hw(x); }
 
 // Level 1, FP32.  kchi = sqrt(chi2_max (1 + CUT_REL)) rounded up.  The test accepts when
 // dist(0, axis segment) <= rho + kchi sqrt(lam_s + lam_c), compared as squares with relative margin 4e-5

@@ -86,23 +86,48 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ------------------------------------------------------------------------------------------- FP64 nominal
-// Acceleration with x^1.5 as u*sqrt(u) and the halo powers from one log2 and two exp2; every operation in
-// FP64 (~1 ulp from the reference's pow-based evaluation).
-__device__ __forceinline__ V3 accel_fast64(const Pot &P, V3 p) {
+// 1 / sqrt(u) and 1 / b in FP64 from the hardware estimates (about 2^-29 relative) and two Newton steps each,
+// within ~1 ulp; u > 0, b != 0.
+__device__ __forceinline__ double rsq64(double u) {
+    double y = __builtin_amdgcn_rsq(u);
+    const double h = 0.5 * u;
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double e = __builtin_fma(-h * y, y, 0.5);   // 0.5 (1 - u y^2)
+        y = __builtin_fma(y, e, y);
+    }
+    return y;
+}
+__device__ __forceinline__ double rcp64(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double e = __builtin_fma(-b, y, 1.0);
+        y = __builtin_fma(y, e, y);
+    }
+    return y;
+}
+
+// Acceleration with every root and quotient through rsq64 / rcp64 (u^-1.5 as (u^-1/2)^3) and the halo powers
+// from one log2 and two exp2; every operation in FP64 (a few ulp from the reference's pow-based evaluation).
+// inv_ha = 1 / halo_a.
+__device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
     const double R2 = p.x * p.x + p.y * p.y, r2 = R2 + p.z * p.z;
-    const double u = r2 + P.bulge_b * P.bulge_b;
-    const double fb = -P.bulge_gm / (u * sqrt(u));
-    const double zeta = sqrt(p.z * p.z + P.disk_b * P.disk_b);
-    const double az = P.disk_a + zeta;
-    const double D2 = R2 + az * az;
-    const double fd = -P.disk_gm / (D2 * sqrt(D2));
-    const double r = sqrt(r2);
-    const double rs = r > 1e-12 ? r : 1e-12;
-    const double xh = (rs < P.halo_cut ? rs : P.halo_cut) / P.halo_a;
+    const double iu = rsq64(r2 + P.bulge_b * P.bulge_b);
+    const double fb = -P.bulge_gm * (iu * iu * iu);
+    const double zz = p.z * p.z + P.disk_b * P.disk_b;
+    const double izeta = rsq64(zz);
+    const double az = P.disk_a + zz * izeta;
+    const double iD = rsq64(R2 + az * az);
+    const double fd = -P.disk_gm * (iD * iD * iD);
+    const double ir = r2 > 0.0 ? rsq64(r2) : 0.0;
+    const double r = r2 * ir;
+    const double irs = r > 1e-12 ? ir : 1e12;
+    const double xh = (r > 1e-12 ? (r < P.halo_cut ? r : P.halo_cut) : (1e-12 < P.halo_cut ? 1e-12 : P.halo_cut)) * inv_ha;
     const double lx = log2(xh);
-    const double gm = P.halo_gm * exp2(P.halo_gamma * lx) / (1.0 + exp2((P.halo_gamma - 1.0) * lx));
-    const double fh = -gm / (rs * rs * rs);
-    return v3((fb + fd + fh) * p.x, (fb + fd + fh) * p.y, (fb + fh) * p.z + fd * p.z * (az / zeta));
+    const double gm = P.halo_gm * exp2(P.halo_gamma * lx) * rcp64(1.0 + exp2((P.halo_gamma - 1.0) * lx));
+    const double fh = -gm * (irs * irs * irs);
+    return v3((fb + fd + fh) * p.x, (fb + fd + fh) * p.y, (fb + fh) * p.z + fd * p.z * (az * izeta));
 }
 
 // ------------------------------------------------------------------------------------------- tracks
@@ -128,7 +153,8 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
             dx[i][c] = (q == 0 && c == i) ? eps : 0.0;
             dv[i][c] = (q == 1 && c == i) ? eps : 0.0;
         }
-    V3 a = accel_fast64(P, x);
+    const double inv_ha = 1.0 / P.halo_a;
+    V3 a = accel_fast64(P, inv_ha, x);
     F3 da[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) da[i] = dforce32(Q, (float)x.x, (float)x.y, (float)x.z, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
@@ -145,7 +171,7 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
                 for (int i = 0; i < 3; i++)
 #pragma unroll
                     for (int c = 0; c < 3; c++) dx[i][c] += dv[i][c] * dt;
-                a = accel_fast64(P, x);
+                a = accel_fast64(P, inv_ha, x);
                 const float xf = (float)x.x, yf = (float)x.y, zf = (float)x.z;
 #pragma unroll
                 for (int i = 0; i < 3; i++) da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);

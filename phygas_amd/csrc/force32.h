@@ -30,6 +30,7 @@ PHX_HDF float f_log2(float x) { return __builtin_amdgcn_logf(x); }
 PHX_HDF float f_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 PHX_HDF float f_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 PHX_HDF float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+PHX_HDF float f_sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }   // the instruction alone (~1 ulp)
 #else
 PHX_HDF float f_rsqrt(float x) { return 1.0f / sqrtf(x); }
 PHX_HDF float f_sqrt(float x) { return sqrtf(x); }
@@ -37,6 +38,7 @@ PHX_HDF float f_log2(float x) { return log2f(x); }
 PHX_HDF float f_exp2(float x) { return exp2f(x); }
 PHX_HDF float f_fma(float a, float b, float c) { return fmaf(a, b, c); }
 PHX_HDF float f_rcp(float x) { return 1.0f / x; }
+PHX_HDF float f_sqrt_hw(float x) { return sqrtf(x); }
 #endif
 // a / b through the hardware reciprocal (within ~2 ulp; a reciprocal shared by several quotients is
 // evaluated once)
@@ -118,7 +120,7 @@ PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy
     const float dg = g0 * pw1(f_div(dr2, u0), -1.5f);
     const float g1 = g0 + dg;
     // disk: -GM (x, y, z (a + zeta) / zeta) D^-3, D^2 = R^2 + (a + zeta)^2, zeta = sqrt(z^2 + b^2)
-    const float zeta0 = f_sqrt(f_fma(z, z, Q.db2)), zeta1 = f_sqrt(f_fma(z + dz, z + dz, Q.db2));
+    const float zeta0 = f_sqrt_hw(f_fma(z, z, Q.db2)), zeta1 = f_sqrt_hw(f_fma(z + dz, z + dz, Q.db2));
     const float dzeta = f_div(dz * (2.0f * z + dz), zeta0 + zeta1);
     const float A0 = Q.disk_a + zeta0;
     const float D20 = f_fma(A0, A0, R2);

@@ -43,6 +43,9 @@ struct DBuf {
 };
 
 constexpr int MAX_STAGES = 12;
+// A fast pass over a batch of at least 2 SPLIT_MIN stars runs as up to MAX_SPLIT sub-passes (contiguous star
+// ranges) on their own streams, so that one sub-pass's latency-bound tracks overlap another's scan and items.
+constexpr int MAX_SPLIT = 4, SPLIT_MIN = 2048;
 
 }  // namespace
 
@@ -69,7 +72,14 @@ struct PhxCtx {
     int last_form = -1;
     double last_chi2_max = 25.0, last_p_min = 0.0;
     int last_n_samp = 64;
-    FastWork fw;       // fast-form work buffers
+    FastWork fw;       // fast-form work buffers (part 0 of a split pass)
+    int split = 2;     // sub-passes of a large fast pass (phx_set_split)
+    int nparts = 1;    // sub-passes of the last fast pass, over stars [part_base[i], part_base[i + 1])
+    int part_base[MAX_SPLIT + 1] = {};
+    long part_rows[MAX_SPLIT] = {};
+    FastWork pfw[MAX_SPLIT];                 // parts 1.. (pfw[0] unused)
+    hipStream_t part_stream[MAX_SPLIT] = {}; // their streams (part_stream[0] unused)
+    hipEvent_t ev_fork = nullptr, ev_join[MAX_SPLIT] = {};
     // timing
     bool timing = false;
     int nstage = 0;
@@ -79,6 +89,9 @@ struct PhxCtx {
     DBuf pps, psteps, pdts, pst, pres, poff, pm, pr50, prt, pout;
     DBuf ptg, pcl, pti, pci, pseed, ppst, ppdt;   // device-side draws
 };
+
+static FastWork &part_fw(PhxCtx *c, int i) { return i == 0 ? c->fw : c->pfw[i]; }
+static hipStream_t part_st(PhxCtx *c, int i) { return i == 0 ? c->st : c->part_stream[i]; }
 
 static uint64_t fnv_str(uint64_t h, const char *s) {
     for (const unsigned char *p = (const unsigned char *)s; *p; p++) h = (h ^ *p) * FNV_PRIME;
@@ -128,6 +141,13 @@ extern "C" void phx_destroy(PhxCtx *c) {
                     &c->prt, &c->pout, &c->ptg, &c->pcl, &c->pti, &c->pci, &c->pseed, &c->ppst, &c->ppdt};
     for (DBuf *b : bufs) b->release();
     fast_release(c->fast, c->fw);
+    for (int i = 1; i < MAX_SPLIT; i++) {
+        if (c->part_stream[i]) (void)hipStreamSynchronize(c->part_stream[i]);
+        fast_release_work(c->pfw[i]);
+        if (c->part_stream[i]) (void)hipStreamDestroy(c->part_stream[i]);
+        if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     for (int i = 0; i <= MAX_STAGES; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     (void)hipStreamDestroy(c->st);
@@ -145,9 +165,18 @@ extern "C" void phx_set_timing(PhxCtx *c, int on) { c->timing = on != 0; }
 
 extern "C" void phx_set_graphs(PhxCtx *c, int on) { c->fw.use_graphs = on != 0; }
 
+extern "C" int phx_set_split(PhxCtx *c, int parts) {
+    PHX_REQUIRE(parts >= 1 && parts <= MAX_SPLIT, "split must be 1..4");
+    c->split = parts;
+    return 0;
+}
+
 extern "C" int phx_graph_stats(PhxCtx *c, long *captures, long *replays) {
-    *captures = c->fw.captures;
-    *replays = c->fw.replays;
+    *captures = *replays = 0;
+    for (int i = 0; i < MAX_SPLIT; i++) {
+        *captures += part_fw(c, i).captures;
+        *replays += part_fw(c, i).replays;
+    }
     return 0;
 }
 
@@ -345,7 +374,36 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     a.group = c->sgroup.as<int>(); a.cl = c->cl.as<PhCluster>(); a.csoa = c->csoa.as<double>();
     a.cn = c->cn.as<int>(); a.R2 = c->R2.as<double>(); a.r50 = c->r50.as<double>(); a.rt = c->rt.as<double>();
     auto mark = [](void *ctx, const char *name) { stage_mark((PhxCtx *)ctx, name); };
-    return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
+    const int ns = a.ns;
+    int parts = c->timing || ns < 2 * SPLIT_MIN ? 1 : std::min(c->split, MAX_SPLIT);
+    c->nparts = parts;
+    for (int i = 0; i <= parts; i++) c->part_base[i] = (int)((long)ns * i / parts);
+    if (parts == 1) return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
+    if (!c->ev_fork) PHX_CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    for (int i = 1; i < parts; i++) {
+        if (!c->part_stream[i]) PHX_CK(hipStreamCreateWithFlags(&c->part_stream[i], hipStreamNonBlocking));
+        if (!c->ev_join[i]) PHX_CK(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+    }
+    PHX_CK(hipEventRecord(c->ev_fork, c->st));   // uploads and cluster tables first
+    for (int i = 1; i < parts; i++) PHX_CK(hipStreamWaitEvent(c->part_stream[i], c->ev_fork, 0));
+    for (int i = 0; i < parts; i++) {
+        const int b = c->part_base[i];
+        FastArgs ai = a;
+        ai.ns = c->part_base[i + 1] - b;
+        ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b;
+        FastWork &W = part_fw(c, i);
+        if (i > 0) {   // the launch settings of part 0
+            W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;
+            W.reduce_blocks = c->fw.reduce_blocks; W.finish_blocks = c->fw.finish_blocks;
+            W.use_graphs = c->fw.use_graphs;
+        }
+        if (fast_screen(c->fast, W, ai, part_st(c, i), nullptr, nullptr)) return 1;
+    }
+    for (int i = 1; i < parts; i++) {   // the context's stream waits for every part
+        PHX_CK(hipEventRecord(c->ev_join[i], c->part_stream[i]));
+        PHX_CK(hipStreamWaitEvent(c->st, c->ev_join[i], 0));
+    }
+    return 0;
 }
 
 extern "C" int phx_screen(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min) {
@@ -381,7 +439,16 @@ static int last_counts(PhxCtx *c, long *kept, long *nrows) {
             }
         } else {
             int grow = 0;
-            if (fast_counts(c->fw, c->st, h, &grow)) return 1;
+            h[0] = h[1] = 0;
+            for (int i = 0; i < c->nparts; i++) {   // the stream has waited for every part
+                long hp[2];
+                int g = 0;
+                if (fast_counts(part_fw(c, i), c->st, hp, &g)) return 1;
+                grow |= g;
+                h[0] += hp[0];
+                h[1] += hp[1];
+                c->part_rows[i] = hp[1];
+            }
             if (grow) {
                 if (screen_pass(c, c->last_form, c->last_chi2_max, c->last_n_samp, c->last_p_min)) return 1;
                 continue;
@@ -453,9 +520,11 @@ extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
         long tiles = 0;
         for (int i = 0; i < c->nc; i++) tiles += (c->cn_h[i] + 7) / 8;
         v[0] = ns * tiles;
-        v[1] = c->fw.last_l2;
-        v[2] = c->fw.last_cand;
-        v[3] = c->fw.last_items;
+        for (int i = 0; i < c->nparts; i++) {
+            v[1] += part_fw(c, i).last_l2;
+            v[2] += part_fw(c, i).last_cand;
+            v[3] += part_fw(c, i).last_items;
+        }
     }
     for (int i = 0; i < cap && i < 6; i++) st[i] = v[i];
     return 0;
@@ -464,7 +533,12 @@ extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
 extern "C" int phx_candidates(PhxCtx *c, long *n) {
     long kept, r;
     if (last_counts(c, &kept, &r)) return 1;
-    *n = c->last_form == PHX_FORM_PLAIN ? (long)c->st_h.size() * c->nc : c->fw.last_cand;
+    if (c->last_form == PHX_FORM_PLAIN) {
+        *n = (long)c->st_h.size() * c->nc;
+    } else {
+        *n = 0;
+        for (int i = 0; i < c->nparts; i++) *n += part_fw(c, i).last_cand;
+    }
     return 0;
 }
 
@@ -473,11 +547,22 @@ extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
     if (last_counts(c, &kept, &nr)) return 1;
     *n = nr;
     long m = std::min(cap, nr);
-    if (m > 0) {
-        const void *src = c->last_form == PHX_FORM_PLAIN ? c->rows.p : fast_rows_ptr(c->fw);
-        PHX_CK(hipMemcpyAsync(rows, src, sizeof(PhxRow) * m, hipMemcpyDeviceToHost, c->st));
+    if (m > 0 && c->last_form == PHX_FORM_PLAIN) {
+        PHX_CK(hipMemcpyAsync(rows, c->rows.p, sizeof(PhxRow) * m, hipMemcpyDeviceToHost, c->st));
         PHX_CK(hipStreamSynchronize(c->st));
-        if (c->last_form != PHX_FORM_PLAIN) std::sort(rows, rows + m, [](const PhxRow &a, const PhxRow &b) {
+    } else if (m > 0) {
+        // each part's rows (its star indices are local to the part), then in (star, cluster) order
+        long o = 0;
+        for (int i = 0; i < c->nparts && o < m; i++) {
+            const long mi = std::min(c->part_rows[i], m - o);
+            if (mi <= 0) continue;
+            PHX_CK(hipMemcpyAsync(rows + o, fast_rows_ptr(part_fw(c, i)), sizeof(PhxRow) * mi, hipMemcpyDeviceToHost,
+                                  c->st));
+            PHX_CK(hipStreamSynchronize(c->st));
+            for (long r = o; r < o + mi; r++) rows[r].star += c->part_base[i];
+            o += mi;
+        }
+        std::sort(rows, rows + m, [](const PhxRow &a, const PhxRow &b) {
             return a.star != b.star ? a.star < b.star : a.cluster < b.cluster;
         });
     }
@@ -488,9 +573,18 @@ extern "C" int phx_star_tracks(PhxCtx *c, int first, int count, double *trk, int
     PHX_REQUIRE(c->last_form >= 0, "no pass has run");
     PHX_REQUIRE(npts_cap == c->npts, "npts_cap must equal phx_npts()");
     PHX_REQUIRE(first >= 0 && first + count <= (int)c->st_h.size(), "star range out of the batch");
-    const double *src = c->last_form == PHX_FORM_PLAIN ? c->trk.as<double>() : fast_tracks_ptr(c->fw);
-    PHX_CK(hipMemcpyAsync(trk, src + (size_t)first * c->npts * 12,
-                          sizeof(double) * 12 * (size_t)c->npts * count, hipMemcpyDeviceToHost, c->st));
+    if (c->last_form == PHX_FORM_PLAIN) {
+        PHX_CK(hipMemcpyAsync(trk, c->trk.as<double>() + (size_t)first * c->npts * 12,
+                              sizeof(double) * 12 * (size_t)c->npts * count, hipMemcpyDeviceToHost, c->st));
+    } else {
+        for (int i = 0; i < c->nparts; i++) {   // the parts' pieces of [first, first + count)
+            const int lo = std::max(first, c->part_base[i]), hi = std::min(first + count, c->part_base[i + 1]);
+            if (lo >= hi) continue;
+            PHX_CK(hipMemcpyAsync(trk + (size_t)(lo - first) * c->npts * 12,
+                                  fast_tracks_ptr(part_fw(c, i)) + (size_t)(lo - c->part_base[i]) * c->npts * 12,
+                                  sizeof(double) * 12 * (size_t)c->npts * (hi - lo), hipMemcpyDeviceToHost, c->st));
+        }
+    }
     PHX_CK(hipStreamSynchronize(c->st));
     return 0;
 }

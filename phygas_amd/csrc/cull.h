@@ -57,6 +57,10 @@ struct TileRec2 {   // FP64, level 2
     double a[3], b[3], U[6], e, R2, t1, pad;
 };
 
+// Level-1 record of a star slot past the end of a batch (the scan reads whole star blocks): far enough that
+// level 1 rejects it against every cluster tile, with its squares finite in FP32.
+PHX_HDF TileRec1 pad_rec1() { return TileRec1{1e12f, 1e12f, 1e12f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}; }
+
 // Neighbouring floats (finite inputs): one step towards +inf / -inf.
 PHX_HDF float next_up(float f) {
     if (f == 0.0f) return 1.4e-45f;

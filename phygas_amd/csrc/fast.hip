@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ tr
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)n_pad * ntl) return;
     const int s = (int)(id / ntl), j = (int)(id % ntl);
-    TileRec1 a = {1e30f, 1e30f, 1e30f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    TileRec1 a = pad_rec1();
     TileRec2 b = {};
     if (s < n) tile_records(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b);
     r1[id] = a;
@@ -1339,18 +1339,22 @@ int fast_z0_check(uint64_t seed, long n, double *worst, hipStream_t st) {
     return 0;
 }
 
-void fast_release(FastTables &T, FastWork &W) {
+void fast_release_work(FastWork &W) {
     graph_drop(W);
-    for (int i = 0; i < 16; i++) {
-        if (T.mem[i]) (void)hipFree(T.mem[i]);
-        T.mem[i] = nullptr;
-        T.bytes[i] = 0;
-    }
     for (int i = 0; i < 24; i++) {
         if (W.mem[i]) (void)hipFree(W.mem[i]);
         W.mem[i] = nullptr;
         W.bytes[i] = 0;
     }
+}
+
+void fast_release(FastTables &T, FastWork &W) {
+    for (int i = 0; i < 16; i++) {
+        if (T.mem[i]) (void)hipFree(T.mem[i]);
+        T.mem[i] = nullptr;
+        T.bytes[i] = 0;
+    }
+    fast_release_work(W);
 }
 
 // ------------------------------------------------------------------------------------------- clones

@@ -75,6 +75,7 @@ def lib() -> C.CDLL:
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
+        "phx_set_split": (_I, [_P, _I]),
         "phx_prefilter": (_I, [_P, _P, _I, _P, _I, _P]),
     }
     for name, (res, args) in sig.items():
@@ -244,6 +245,10 @@ class Engine:
 
     def set_graphs(self, on: bool = True):
         self._L.phx_set_graphs(self.h, int(on))
+
+    def set_split(self, parts: int):
+        """Sub-passes (1..4) of a fast pass over a large batch, each on its own stream."""
+        _check(self._L.phx_set_split(self.h, int(parts)), "phx_set_split")
 
     def graph_stats(self) -> tuple[int, int]:
         c, r = _L(0), _L(0)

@@ -39,3 +39,20 @@ extern "C" int cc_scan(const double *strk, int ns, int snpts, const double *ctrk
 extern "C" double cc_rot_check(void) {
     return fmax(fabs(ROT_C - cos(OMEGA_PT)), fabs(ROT_S - sin(OMEGA_PT)));
 }
+
+// Level-1 passes of the padding star record against every tile of every cluster (0 expected).
+extern "C" long cc_pad_passes(const double *ctrk, const int *cn, const double *R2, int nc, int cnpts, double chi2_max) {
+    const float kchi = up(sqrt(chi2_max * (1.0 + CUT_REL)));
+    const TileRec1 pad = pad_rec1();
+    long passes = 0;
+    for (int c = 0; c < nc; c++) {
+        const int n = cn[c], nt = (n + TILE - 1) / TILE;
+        for (int j = 0; j < nt; j++) {
+            TileRec1 c1;
+            TileRec2 c2;
+            tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], c1, c2);
+            passes += l1_pass(pad, c1, tile_t1(n, j), kchi) ? 1 : 0;
+        }
+    }
+    return passes;
+}

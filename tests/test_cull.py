@@ -107,6 +107,10 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     f1 = p1[np.broadcast_to(exists, p1.shape)].mean()
     f2 = p2[np.broadcast_to(exists, p2.shape)].mean()
     assert f1 < 0.35 and f2 < 0.12, (f1, f2)
+    # the record of a star slot past a batch's end passes level 1 against no cluster tile
+    harness.cc_pad_passes.restype = C.c_long
+    harness.cc_pad_passes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
+    assert harness.cc_pad_passes(ct.ctypes.data, cn.ctypes.data, R2.ctypes.data, len(cl), npts, CHI2) == 0
 
 
 def test_level2_rotation_constants(harness):

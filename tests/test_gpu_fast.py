@@ -175,6 +175,35 @@ def test_autotune_keeps_rows_and_stats_consistent(unit):
         _report(f"autotune_{k}_blocks", tune[f"{k}_blocks"])
 
 
+def test_split_passes_give_the_same_rows(workload):
+    """A pass split into 2, 3 or 4 sub-passes on their own streams gives the whole pass's rows, work counts and
+    tracks (also with hipGraph replays)."""
+    cl = workload.make_clusters(4087, 0)
+    st = workload.make_stars(2500, 2)
+    ctl = workload.shuffled_control(st, 3)
+    out = {}
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        npts = e.npts
+        for parts in (1, 2, 3, 4):
+            e.set_split(parts)
+            for graphs in (False, True):
+                e.set_graphs(graphs)
+                rows = [e.screen(hip.FAST, chi2_max=25.0, p_min=p) for p in (0.0, 0.001)]
+                out[(parts, graphs)] = (rows, e.pass_stats(), e.star_tracks(1240, 20))
+        e.set_graphs(False)
+    ref_rows, ref_stats, ref_trk = out[(1, False)]
+    assert len(ref_rows[0]) > 1000 and len(ref_rows[1]) > 50
+    assert ref_trk.shape == (20, npts, 12)
+    for key, (rows, stats, trk) in out.items():
+        for a, b in zip(rows, ref_rows):
+            assert a.tobytes() == b.tobytes(), key
+        assert stats == ref_stats, key
+        assert trk.tobytes() == ref_trk.tobytes(), key
+
+
 @pytest.fixture(scope="module")
 def pulsar_unit(workload):
     """1 synthetic pulsar x 300 clusters x 200 clones, dt 0.05 Myr (clones drawn on the host, seed per pair)."""

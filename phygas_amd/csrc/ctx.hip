@@ -517,10 +517,8 @@ extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
         v[0] = ns * c->nc;   // every pair is scanned point by point
         v[2] = ns * c->nc;
     } else {
-        long tiles = 0;
-        for (int i = 0; i < c->nc; i++) tiles += (c->cn_h[i] + 7) / 8;
-        v[0] = ns * tiles;
         for (int i = 0; i < c->nparts; i++) {
+            v[0] += part_fw(c, i).last_l1;
             v[1] += part_fw(c, i).last_l2;
             v[2] += part_fw(c, i).last_cand;
             v[3] += part_fw(c, i).last_items;

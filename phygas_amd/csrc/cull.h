@@ -44,6 +44,7 @@
 namespace phx {
 
 constexpr int TILE = 8;
+constexpr int TILE2 = 16;   // coarse tile of the scan's first pass: two tiles
 constexpr double OMEGA_PT = -0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122;   // frame rotation per 0.5-Myr point
 constexpr double ROT_W = 0.5 * (220.0 / KMS_PER_KPCMYR) / 8.122 * 1.000001;   // |R(w) x - x| <= ROT_W |x|
 constexpr double CUT_REL = 1e-6;
@@ -164,17 +165,19 @@ PHX_HDF double inv_trace(const double *S) {
     return det > 0.0 ? (c0 + c3 + c5) / det : 0.0;
 }
 
-// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).
-PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2) {
-    const int k0 = j * TILE;
-    const int kl = k0 + TILE < n - 1 ? k0 + TILE : n - 1;   // last position
-    const int kp = (k0 + TILE < n ? k0 + TILE : n) - 1;     // last point
+// Axis of tile j (T points) of an object with n points: the line fit and tube radius fields of r1 / r2 and
+// t1; returns the first and last point indices (k0, kp).  trk = its FP64 track [n][12] (pos, vel, S).
+template <int T = TILE>
+PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &r2, int &k0_out, int &kp_out) {
+    const int k0 = j * T;
+    const int kl = k0 + T < n - 1 ? k0 + T : n - 1;   // last position
+    const int kp = (k0 + T < n ? k0 + T : n) - 1;     // last point
     const int m = kl - k0 + 1;
-    const double tc = k0 + 0.5 * TILE;
-    double px[TILE + 1], py[TILE + 1], pz[TILE + 1];
+    const double tc = k0 + 0.5 * T;
+    double px[T + 1], py[T + 1], pz[T + 1];
     double st = 0, stt = 0, sx = 0, sy = 0, sz = 0, stx = 0, sty = 0, stz = 0;
 #pragma unroll
-    for (int i = 0; i <= TILE; i++) {
+    for (int i = 0; i <= T; i++) {
         const int k = k0 + (i < m ? i : m - 1);
         const double *p = trk + (size_t)k * 12;
         const double th = OMEGA_PT * k, c = cos(th), s = sin(th);
@@ -199,7 +202,7 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     const float af[3] = {(float)a[0], (float)a[1], (float)a[2]}, bf[3] = {(float)b[0], (float)b[1], (float)b[2]};
     double e64 = 0.0, e32 = 0.0;
 #pragma unroll
-    for (int i = 0; i <= TILE; i++) {
+    for (int i = 0; i <= T; i++) {
         if (i < m) {
             const double t = (k0 + i) - tc;
             const double rx = px[i] - (a[0] + b[0] * t), ry = py[i] - (a[1] + b[1] * t), rz = pz[i] - (a[2] + b[2] * t);
@@ -209,6 +212,23 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
             e32 = fmax(e32, sqrt(fx * fx + fy * fy + fz * fz));
         }
     }
+    r1.ax = af[0]; r1.ay = af[1]; r1.az = af[2];
+    r1.bx = bf[0]; r1.by = bf[1]; r1.bz = bf[2];
+    r1.e = up(e32 * (1.0 + 1e-6) + 1e-12);
+#pragma unroll
+    for (int u = 0; u < 3; u++) { r2.a[u] = a[u]; r2.b[u] = b[u]; }
+    r2.e = e64 * (1.0 + 1e-12) + 1e-15;
+    r2.t1 = (double)(kl - k0) - 0.5 * T;
+    r2.pad = 0.0;
+    k0_out = k0;
+    kp_out = kp;
+}
+
+// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).
+template <int T = TILE>
+PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2) {
+    int k0, kp;
+    tile_line<T>(trk, n, j, r1, r2, k0, kp);
     // Loewner bounds of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
     // and, for tiles of more than one point, sum + eta2 I (sum = the first point's + the last point's; over
     // a tile the covariance grows nearly quadratically in time, so every point's lies close to or below the
@@ -228,7 +248,7 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     }
     double eta1 = 0.0, eta2 = 0.0, trmax = sum[0] + sum[3] + sum[5];
 #pragma unroll
-    for (int i = 0; i < TILE; i++) {
+    for (int i = 0; i < T; i++) {
         const int k = k0 + i;
         if (k <= kp) {
             const double th = OMEGA_PT * k;
@@ -250,24 +270,52 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     const double U1[6] = {ref[0] + eta1 + R2, ref[1], ref[2], ref[3] + eta1 + R2, ref[4], ref[5] + eta1 + R2};
     const double U2[6] = {sum[0] + eta2 + R2, sum[1], sum[2], sum[3] + eta2 + R2, sum[4], sum[5] + eta2 + R2};
     const double *U = kp > k0 && inv_trace(U2) > inv_trace(U1) ? U2 : U1;
-    r1.ax = af[0]; r1.ay = af[1]; r1.az = af[2];
-    r1.bx = bf[0]; r1.by = bf[1]; r1.bz = bf[2];
-    r1.e = up(e32 * (1.0 + 1e-6) + 1e-12);
     r1.lam = up(fmin(lam_bound(U1), lam_bound(U2)) * (1.0 + 1e-6));
 #pragma unroll
-    for (int u = 0; u < 3; u++) { r2.a[u] = a[u]; r2.b[u] = b[u]; }
+    for (int u = 0; u < 6; u++) r2.U[u] = U[u];
+    r2.R2 = R2;
+}
+
+// Records of the coarse tile J (TILE2 = 2 TILE points) from the records fa, fb of its tiles 2J, 2J + 1 (fb null
+// when the track ends in tile 2J): the axis fitted to its own positions, and U = Ua + (Ub - Ua)_+ + eta I,
+// eta >= the largest eigenvalues of Ua - U and Ub - U, so that U >= Ua, Ub, which bound every point of the two
+// tiles (their kernel term included).
+PHX_HDF void coarse_records(const double *trk, int n, int J, const TileRec2 &fa, const TileRec2 *fb, TileRec1 &r1,
+                            TileRec2 &r2) {
+    int k0, kp;
+    tile_line<TILE2>(trk, n, J, r1, r2, k0, kp);
+    double U[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) U[u] = fa.U[u];
+    if (fb) {
+        double d[6], pp[6];
+#pragma unroll
+        for (int u = 0; u < 6; u++) d[u] = fb->U[u] - fa.U[u];
+        sym3_pos_part(d, pp);
+#pragma unroll
+        for (int u = 0; u < 6; u++) U[u] += pp[u];
+        double da[6], db[6];
+#pragma unroll
+        for (int u = 0; u < 6; u++) {
+            da[u] = fa.U[u] - U[u];
+            db[u] = fb->U[u] - U[u];
+        }
+        const double tr = fmax(U[0] + U[3] + U[5], fb->U[0] + fb->U[3] + fb->U[5]);
+        const double eta = fmax(0.0, fmax(sym3_lmax_bound(da), sym3_lmax_bound(db))) * (1.0 + 1e-12) + 1e-13 * tr;
+        U[0] += eta; U[3] += eta; U[5] += eta;
+    }
+    r1.lam = up(lam_bound(U) * (1.0 + 1e-6));
 #pragma unroll
     for (int u = 0; u < 6; u++) r2.U[u] = U[u];
-    r2.e = e64 * (1.0 + 1e-12) + 1e-15;
-    r2.R2 = R2;
-    r2.t1 = (double)(kl - k0) - 0.5 * TILE;
-    r2.pad = 0.0;
+    r2.R2 = fb ? fmin(fa.R2, fb->R2) : fa.R2;
+    (void)k0; (void)kp;
 }
 
 // The axis parameter range of tile j of a cluster with n points: t in [-4, t1].
+template <int T = TILE>
 PHX_HDF float tile_t1(int n, int j) {
-    const int k0 = j * TILE, kl = k0 + TILE < n - 1 ? k0 + TILE : n - 1;
-    return (float)(kl - k0) - 0.5f * TILE;
+    const int k0 = j * T, kl = k0 + T < n - 1 ? k0 + T : n - 1;
+    return (float)(kl - k0) - 0.5f * T;
 }
 
 // Square root to within 1 ulp (the hardware instruction on the device).
@@ -277,10 +325,11 @@ PHX_HDF float f_sqrt_fast(float x) { return f_sqrt_hw(x); }
 // dist(0, axis segment) <= rho + kchi sqrt(lam_s + lam_c), compared as squares with relative margin 4e-5
 // and an absolute one of 1e-5 kpc (plus 2e-6 |da|^2 for the cancellation in the cross product); FP32
 // rounding of every step is far below both.
+template <int T = TILE>
 PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi) {
     const float dax = s.ax - c.ax, day = s.ay - c.ay, daz = s.az - c.az;
     const float dbx = s.bx - c.bx, dby = s.by - c.by, dbz = s.bz - c.bz;
-    const float t0 = -0.5f * TILE;
+    const float t0 = -0.5f * T;
     const float dd = f_fma(dbx, dbx, f_fma(dby, dby, dbz * dbz));
     const float ad = f_fma(dax, dbx, f_fma(day, dby, daz * dbz));
     const float x0x = f_fma(dbx, t0, dax), x0y = f_fma(dby, t0, day), x0z = f_fma(dbz, t0, daz);
@@ -292,7 +341,7 @@ PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi)
     const bool interior = (dd > 0.0f) & (-ad > t0 * dd) & (-ad < t1 * dd);
     const float e = s.e + c.e;
     const float na = fabsf(dax) + fabsf(day) + fabsf(daz), nb = fabsf(dbx) + fabsf(dby) + fabsf(dbz);
-    const float rho = e + (float)ROT_W * (na + 0.5f * TILE * nb + e);
+    const float rho = e + (float)ROT_W * (na + 0.5f * T * nb + e);
     const float lim = (rho + kchi * f_sqrt_fast(s.lam + c.lam)) * 1.00001f + 1e-5f;
     const float lim2 = f_fma(lim, lim, 2e-6f * na * na);
     const float lhs = interior ? dl : fminf(d0, d1);
@@ -306,6 +355,7 @@ PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi)
 // of the axis segment) up to a positive factor, which needs no division; any z is valid, so only the
 // rounding of L and z^T U z is bounded (err_l, err_u).  U is positive definite: its diagonal holds the
 // kernel term R^2 > 0 and the trace guard bounds its condition number.
+template <int T = TILE>
 PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi2c) {
     const double da0 = s.a[0] - c.a[0], da1 = s.a[1] - c.a[1], da2 = s.a[2] - c.a[2];
     const double db0 = s.b[0] - c.b[0], db1 = s.b[1] - c.b[1], db2 = s.b[2] - c.b[2];
@@ -324,7 +374,7 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     const double gb0 = G0 * db0 + G1 * db1 + G2 * db2, gb1 = G1 * db0 + G3 * db1 + G4 * db2,
                  gb2 = G2 * db0 + G4 * db1 + G5 * db2;
     const double B = da0 * gb0 + da1 * gb1 + da2 * gb2, C = db0 * gb0 + db1 * gb1 + db2 * gb2;
-    const double t0 = -0.5 * TILE;
+    const double t0 = -0.5 * T;
     // z = C (ga + th gb) with th = -B / C inside [t0, t1], else ga + t gb at the nearer end
     double wa = C, wb = -B;
     if (!(C > 0.0) || -B <= t0 * C) { wa = 1.0; wb = t0; }
@@ -342,7 +392,7 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
                  u2 = U[2] * z0 + U[4] * z1 + U[5] * z2;
     const double zuz = z0 * u0 + z1 * u1 + z2 * u2;
     const double na1 = fabs(da0) + fabs(da1) + fabs(da2), nb1 = fabs(db0) + fabs(db1) + fabs(db2);
-    const double err_l = 1e-12 * nz * (na1 + 0.5 * TILE * nb1 + rho);
+    const double err_l = 1e-12 * nz * (na1 + 0.5 * T * nb1 + rho);
     const double err_u = 1e-12 * tr * nz2;
     const double L = lin - rho * nz - err_l;
     return !(L > 0.0 && L * L > chi2c * (zuz + err_u) * (1.0 + 1e-12));

@@ -57,16 +57,16 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
-       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_NSLOT };
+       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SREC2C, W_NSLOT };
 // Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
-// for the exact evaluation by the finish screen
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_N };
+// for the exact evaluation by the finish screen, largest kept segment, level-1 tests of the scan
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
 // The reduce appends kept pairs to KSEG segments of the W_KSEG list, each with its own counter (counter i at
 // C_KSEG0 + i KSEG_STRIDE, one per 256 bytes): one counter for every wavefront of the grid would take its
 // atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
 constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32, C_TOTAL = C_KSEG0 + KSEG * KSEG_STRIDE;
 // Table buffers (FastTables::mem slots)
-enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_NSLOT };
+enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CREC2C, T_NSLOT };
 
 static int ensure(void *&p, size_t &have, size_t need) {
     if (need <= have && p) return 0;
@@ -238,6 +238,24 @@ __global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ tr
     r2[id] = b;
 }
 
+// Coarse records (TILE2 points: tiles 2J, 2J + 1) of the stars from their tile records r2 [n_pad][ntl]:
+// rc1 / rc2 [n_pad][ntlc]; padding rows as k_star_recs's.
+__global__ __launch_bounds__(256) void k_star_recs_coarse(const double *__restrict__ trk, int npts, int ntl, int ntlc,
+                                                          int n, int n_pad, const TileRec2 *__restrict__ r2,
+                                                          TileRec1 *__restrict__ rc1, TileRec2 *__restrict__ rc2) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)n_pad * ntlc) return;
+    const int s = (int)(id / ntlc), J = (int)(id % ntlc);
+    TileRec1 a = pad_rec1();
+    TileRec2 b = {};
+    if (s < n) {
+        const TileRec2 *f = r2 + (size_t)s * ntl + 2 * J;
+        coarse_records(trk + (size_t)s * npts * 12, npts, J, f[0], 2 * J + 1 < ntl ? f + 1 : nullptr, a, b);
+    }
+    rc1[id] = a;
+    rc2[id] = b;
+}
+
 // Cluster tables for the fast form: AoS FP64 tracks [c][npts][12].
 __global__ __launch_bounds__(256) void k_cluster_aos(const double *__restrict__ csoa, const int *__restrict__ cn,
                                                      int nc, int npts, double *__restrict__ ctrk) {
@@ -271,35 +289,81 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
     r2[id] = b;
 }
 
+// Coarse records of the clusters from their tile records r2 [ntl][nc_pad] (sorted order): rc1 / rc2
+// [ntlc][nc_pad].
+__global__ __launch_bounds__(256) void k_cluster_recs_coarse(const double *__restrict__ ctrk, const int *__restrict__ cn,
+                                                             const int *__restrict__ perm, int nc, int nc_pad, int npts,
+                                                             int ntlc, const TileRec2 *__restrict__ r2,
+                                                             TileRec1 *__restrict__ rc1, TileRec2 *__restrict__ rc2) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nc_pad * ntlc) return;
+    const int J = (int)(id / nc_pad), cs = (int)(id % nc_pad);
+    TileRec1 a = {};
+    TileRec2 b = {};
+    if (cs < nc) {
+        const int c = perm[cs], n = cn[c];
+        if (2 * J * TILE < n)
+            coarse_records(ctrk + (size_t)c * npts * 12, n, J, r2[(size_t)(2 * J) * nc_pad + cs],
+                           (2 * J + 1) * TILE < n ? r2 + (size_t)(2 * J + 1) * nc_pad + cs : nullptr, a, b);
+    }
+    rc1[id] = a;
+    rc2[id] = b;
+}
+
 // ------------------------------------------------------------------------------------------- tile scan
 // Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) from a
-// counter until none is left, lane = cluster.  Per tile j:
-//   level 1  every star of the block against the lane's cluster (the star's record is wave-uniform and read
-//            through scalar loads);
+// counter until none is left, lane = cluster.  Per coarse tile J (TILE2 = 2 TILE points: tiles 2J, 2J + 1):
+//   level 1  every star of the block against the lane's cluster, on the coarse records (the star's record
+//            is wave-uniform and read through scalar loads);
 //   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
-//            evaluated 64 at a time, one per lane, with the cluster records of the tile staged in LDS; the
-//            verdicts return to the cluster's lane as star bits.
-// Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64 tiles).
+//            evaluated 64 at a time, one per lane, with the cluster's coarse records staged in LDS;
+//   fine     the coarse tiles left, queued again, each entry testing tiles 2J and 2J + 1 with level 2 on
+//            their own records (level 1 would pass nearly all of them).  The verdicts return to the
+//            cluster's lane as (star, tile) bits.
+// A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
+// drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
 template <typename MaskT>
-__global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__ crec1,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
+                                                    const TileRec2 *__restrict__ crec2c,
                                                     const TileRec2 *__restrict__ crec2,
                                                     const int *__restrict__ ntc_sorted,
                                                     const int *__restrict__ cn_sorted, const int *__restrict__ perm,
-                                                    int nc, int nc_pad, const TileRec1 *__restrict__ srec1,
-                                                    const TileRec2 *__restrict__ srec2, int ns, int ntl,
+                                                    int nc, int nc_pad, const TileRec1 *__restrict__ srec1c,
+                                                    const TileRec2 *__restrict__ srec2c,
+                                                    const TileRec2 *__restrict__ srec2, int ns, int ntl, int ntlc,
                                                     float kchi, double chi2c, Cand *__restrict__ cand,
                                                     unsigned long long *__restrict__ ncand, long cap,
                                                     unsigned long long *__restrict__ next_task,
+                                                    unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
     __shared__ unsigned short q_s[4][SPW * 64];
     __shared__ TileRec2 c2_s[4][64];
     __shared__ float t1_s[4][64];
     __shared__ unsigned l2_s[4][64];
+    __shared__ int nt_s[4][64], ncl_s[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned short *q = q_s[w];
     TileRec2 *c2s = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
     const int ntasks = ncb * nsb;
+    // exclusive prefix of a per-lane count over the wavefront; the total into *tot
+    auto excl_prefix = [&](unsigned cnt, int *tot) {
+        unsigned x = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        *tot = __builtin_amdgcn_readfirstlane(__shfl(x, 63, 64));
+        return x - cnt;
+    };
+    auto enqueue = [&](unsigned bits, unsigned o) {
+        while (bits) {
+            const int i = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            q[o++] = (unsigned short)(i << 6 | lane);
+        }
+    };
     for (;;) {
         // tasks in star-block-major order, taken dynamically: the waves in flight share few star blocks
         int t = 0;
@@ -312,45 +376,37 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
         const int ncl = cs < nc ? cn_sorted[cs] : 0;
         const int ntmax = __builtin_amdgcn_readfirstlane(nt);   // sorted descending: lane 0 has the longest
         const int s0 = sb * SPW;
-        unsigned n_l2 = 0;   // level-2 tests of this task (wave-uniform)
+        const int nsv = min(SPW, ns - s0);   // stars of the block (the rest is padding)
+        nt_s[w][lane] = nt;
+        ncl_s[w][lane] = ncl;
+        unsigned n_l1 = 0, n_l2 = 0;   // tests of this task (wave-uniform)
+        unsigned fine_l2 = 0;          // fine level-2 tests of this lane's queue entries
         MaskT mask[SPW];
 #pragma unroll
         for (int i = 0; i < SPW; i++) mask[i] = 0;
-        for (int j = 0; j < ntmax; j++) {
-            const bool cv = j < nt;
-            const TileRec1 c1 = crec1[(size_t)j * nc_pad + cs];
-            const float t1 = cv ? tile_t1(ncl, j) : 0.0f;
+        for (int J = 0; 2 * J < ntmax; J++) {
+            const bool cv = 2 * J < nt;
+            const TileRec1 c1 = crec1c[(size_t)J * nc_pad + cs];
+            const float t1 = cv ? tile_t1<TILE2>(ncl, J) : 0.0f;
+            n_l1 += (unsigned)nsv * (unsigned)__popcll(__ballot(cv));
             unsigned l1 = 0u;
 #pragma unroll
             for (int i0 = 0; i0 < SPW; i0 += 8) {   // star records of the padded block: wave-uniform loads
                 TileRec1 sr[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) sr[u] = srec1[(size_t)(s0 + i0 + u) * ntl + j];
+                for (int u = 0; u < 8; u++) sr[u] = srec1c[(size_t)(s0 + i0 + u) * ntlc + J];
 #pragma unroll
                 for (int u = 0; u < 8; u++)
-                    if (l1_pass(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
+                    if (l1_pass<TILE2>(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
             }
             if (!cv) l1 = 0u;
-            // queue the wavefront's level-1 survivors: exclusive prefix of the per-lane counts
-            const unsigned cnt = (unsigned)__popc(l1);
-            unsigned x = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const unsigned y = __shfl_up(x, d, 64);
-                if (lane >= d) x += y;
-            }
-            const int nq = __builtin_amdgcn_readfirstlane(__shfl(x, 63, 64));
+            // coarse level 2 over the wavefront's level-1 survivors
+            int nq = 0;
+            const unsigned o1 = excl_prefix((unsigned)__popc(l1), &nq);
             if (nq == 0) continue;
             n_l2 += nq;
-            {
-                unsigned o = x - cnt, b = l1;
-                while (b) {
-                    const int i = __builtin_ctz(b);
-                    b &= b - 1u;
-                    q[o++] = (unsigned short)(i << 6 | lane);
-                }
-            }
-            if (cnt) c2s[lane] = crec2[(size_t)j * nc_pad + cs];
+            enqueue(l1, o1);
+            if (l1) c2s[lane] = crec2c[(size_t)J * nc_pad + cs];
             t1_s[w][lane] = t1;
             l2_s[w][lane] = 0u;
             wave_sync();
@@ -358,17 +414,51 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const TileRec1 *__restrict__
                 const int e = e0 + lane;
                 if (e < nq) {
                     const int ent = q[e], i = ent >> 6, c = ent & 63;
-                    const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
-                    if (l2_pass(s2, c2s[c], (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
+                    const TileRec2 s2 = srec2c[(size_t)(s0 + i) * ntlc + J];
+                    if (l2_pass<TILE2>(s2, c2s[c], (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
+                }
+            }
+            wave_sync();
+            const unsigned lc = l2_s[w][lane];
+            wave_sync();   // q / l2_s are rewritten below
+            // fine level 2 on tiles 2J, 2J + 1 of the coarse tiles left
+            int nq2 = 0;
+            const unsigned o2 = excl_prefix((unsigned)__popc(lc), &nq2);
+            if (nq2 == 0) continue;
+            enqueue(lc, o2);
+            l2_s[w][lane] = 0u;
+            wave_sync();
+            for (int e0 = 0; e0 < nq2; e0 += 64) {   // wave-uniform
+                const int e = e0 + lane;
+                if (e < nq2) {
+                    const int ent = q[e], i = ent >> 6, c = ent & 63;
+                    const int ntc = nt_s[w][c], nclc = ncl_s[w][c], csc = cb * 64 + c;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int j = 2 * J + h;
+                        if (j < ntc) {
+                            fine_l2++;
+                            const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
+                            const TileRec2 c2 = crec2[(size_t)j * nc_pad + csc];
+                            if (l2_pass(s2, c2, (double)tile_t1(nclc, j), chi2c)) atomicOr(&l2_s[w][c], 1u << (2 * i + h));
+                        }
+                    }
                 }
             }
             wave_sync();
             const unsigned l2 = l2_s[w][lane];
 #pragma unroll
-            for (int i = 0; i < SPW; i++) mask[i] |= (MaskT)((l2 >> i) & 1u) << j;
-            wave_sync();   // q / c2s / l2_s are rewritten by the next tile
+            for (int i = 0; i < SPW; i++)
+                mask[i] |= (MaskT)((l2 >> (2 * i)) & 3u) << (2 * J);
+            wave_sync();   // q / c2s / l2_s are rewritten by the next coarse tile
         }
-        if (lane == 0 && n_l2) atomicAdd(n_l2_total, (unsigned long long)n_l2);
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) fine_l2 += __shfl_xor(fine_l2, d, 64);
+        n_l2 += fine_l2;
+        if (lane == 0) {
+            atomicAdd(n_l1_total, (unsigned long long)n_l1);
+            if (n_l2) atomicAdd(n_l2_total, (unsigned long long)n_l2);
+        }
         // the task's candidates take one range of the list (one atomic per task)
         unsigned total = 0;
 #pragma unroll
@@ -1087,6 +1177,7 @@ __global__ void k_zero_counters(unsigned long long *c, int n) {
 
 // ------------------------------------------------------------------------------------------- host side
 static int n_tiles(int npts) { return (npts + TILE - 1) / TILE; }
+static int n_coarse(int ntl) { return (ntl + 1) / 2; }   // coarse tiles (TILE2 = 2 TILE points)
 static int ns_pad(int ns) { return (ns + SPW - 1) / SPW * SPW; }
 
 int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, const double *d_R2, int nc, int npts,
@@ -1115,12 +1206,14 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
         cn_sorted[i] = cn[perm[i]];
     }
     T.nc_pad = (nc + 63) / 64 * 64;
-    const size_t nrec = (size_t)T.ntiles * T.nc_pad;
+    const size_t nrec = (size_t)T.ntiles * T.nc_pad, nrec_c = (size_t)n_coarse(T.ntiles) * T.nc_pad;
     if (ensure(T.mem[T_CNT], T.bytes[T_CNT], 4 * (size_t)T.nc_pad) ||
         ensure(T.mem[T_CPERM], T.bytes[T_CPERM], 4 * (size_t)T.nc_pad) ||
         ensure(T.mem[T_CNS], T.bytes[T_CNS], 4 * (size_t)T.nc_pad) ||
         ensure(T.mem[T_CREC1], T.bytes[T_CREC1], sizeof(TileRec1) * nrec) ||
-        ensure(T.mem[T_CREC2], T.bytes[T_CREC2], sizeof(TileRec2) * nrec))
+        ensure(T.mem[T_CREC2], T.bytes[T_CREC2], sizeof(TileRec2) * nrec) ||
+        ensure(T.mem[T_CREC1C], T.bytes[T_CREC1C], sizeof(TileRec1) * nrec_c) ||
+        ensure(T.mem[T_CREC2C], T.bytes[T_CREC2C], sizeof(TileRec2) * nrec_c))
         return 1;
     nt_sorted.resize(T.nc_pad, 0);
     cn_sorted.resize(T.nc_pad, 0);
@@ -1131,6 +1224,10 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
     hipLaunchKernelGGL(k_cluster_recs, dim3(blocks_for((long)nrec, 256)), dim3(256), 0, st,
                        (const double *)T.mem[T_CTRK], d_cn, d_R2, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
                        T.ntiles, (TileRec1 *)T.mem[T_CREC1], (TileRec2 *)T.mem[T_CREC2]);
+    hipLaunchKernelGGL(k_cluster_recs_coarse, dim3(blocks_for((long)nrec_c, 256)), dim3(256), 0, st,
+                       (const double *)T.mem[T_CTRK], d_cn, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
+                       n_coarse(T.ntiles), (const TileRec2 *)T.mem[T_CREC2], (TileRec1 *)T.mem[T_CREC1C],
+                       (TileRec2 *)T.mem[T_CREC2C]);
     PHX_CK(hipGetLastError());
     PHX_CK(hipStreamSynchronize(st));
     T.d_cn = d_cn;
@@ -1152,6 +1249,8 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
         ensure(W.mem[W_TILED], W.bytes[W_TILED], sizeof(double) * TREC * (size_t)ns * ntl) ||
         ensure(W.mem[W_SREC1], W.bytes[W_SREC1], sizeof(TileRec1) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_SREC2], W.bytes[W_SREC2], sizeof(TileRec2) * (size_t)ns_pad(ns) * ntl) ||
+        ensure(W.mem[W_SREC1C], W.bytes[W_SREC1C], sizeof(TileRec1) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
+        ensure(W.mem[W_SREC2C], W.bytes[W_SREC2C], sizeof(TileRec2) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
         ensure(W.mem[W_CAND], W.bytes[W_CAND], sizeof(Cand) * (size_t)W.cand_cap) ||
         ensure(W.mem[W_KEPT], W.bytes[W_KEPT], sizeof(Kept) * (size_t)W.kept_cap) ||
         ensure(W.mem[W_ROWS], W.bytes[W_ROWS], sizeof(PhxRow) * (size_t)W.row_cap) ||
@@ -1194,15 +1293,19 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     if (mark) mark(mark_ctx, "tracks");
     hipLaunchKernelGGL(k_star_recs, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts, ntl, ns,
                        ns_pad(ns), srec1, srec2);
+    hipLaunchKernelGGL(k_star_recs_coarse, dim3(blocks_for((long)ns_pad(ns) * n_coarse(ntl), 256)), dim3(256), 0, st,
+                       trk, npts, ntl, n_coarse(ntl), ns, ns_pad(ns), (const TileRec2 *)srec2,
+                       (TileRec1 *)W.mem[W_SREC1C], (TileRec2 *)W.mem[W_SREC2C]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
     auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
     hipLaunchKernelGGL(scan, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
-                       (const TileRec1 *)T.mem[T_CREC1], (const TileRec2 *)T.mem[T_CREC2], (const int *)T.mem[T_CNT],
-                       (const int *)T.mem[T_CNS], (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, srec1, srec2, ns, ntl,
-                       up(sqrt(chi2c)), chi2c, (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK,
-                       cnt + C_L2);
+                       (const TileRec1 *)T.mem[T_CREC1C], (const TileRec2 *)T.mem[T_CREC2C],
+                       (const TileRec2 *)T.mem[T_CREC2], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
+                       (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
+                       (const TileRec2 *)W.mem[W_SREC2C], srec2, ns, ntl, n_coarse(ntl), up(sqrt(chi2c)), chi2c,
+                       (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK, cnt + C_L1, cnt + C_L2);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "scan");
     const Cand *cand = (const Cand *)W.mem[W_CAND];
@@ -1308,6 +1411,7 @@ int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
     W.last_cand = (long)c[C_CAND];
     W.last_items = (long)c[C_ITEMS];
     W.last_l2 = (long)c[C_L2];
+    W.last_l1 = (long)c[C_L1];
     *grow = 0;
     if ((long)c[C_CAND] > W.cand_cap) { W.cand_cap = (long)c[C_CAND] + (long)c[C_CAND] / 4 + 4096; *grow = 1; }
     if ((long)c[C_ITEMS] > W.item_cap) { W.item_cap = (long)c[C_ITEMS] + (long)c[C_ITEMS] / 4 + 4096; *grow = 1; }

@@ -107,6 +107,18 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     f1 = p1[np.broadcast_to(exists, p1.shape)].mean()
     f2 = p2[np.broadcast_to(exists, p2.shape)].mean()
     assert f1 < 0.35 and f2 < 0.12, (f1, f2)
+    # the scan's order: coarse tiles (two tiles) through both levels first, then the tiles of those left
+    harness.cc_scan2.restype = C.c_int
+    harness.cc_scan2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                 C.c_double, C.c_void_p, C.c_void_p]
+    p2c = np.zeros_like(p2)
+    counts = np.zeros(4, np.int64)
+    assert harness.cc_scan2(stt.ctypes.data, len(st), npts, ct.ctypes.data, cn.ctypes.data, R2.ctypes.data, len(cl),
+                            npts, CHI2, p2c.ctypes.data, counts.ctypes.data) == ntl
+    assert qt[p2c == 0].min() >= CHI2 * (1 - 1e-9)
+    assert np.all(p2c <= p2)                       # only ever drops more
+    assert np.all(p2c[ks, kc, kb[ks, kc] // 8] == 1)
+    assert counts[2] < 0.6 * np.broadcast_to(exists, p1.shape).sum()   # coarse level-1 tests: about half
     # the record of a star slot past a batch's end passes level 1 against no cluster tile
     harness.cc_pad_passes.restype = C.c_long
     harness.cc_pad_passes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]

@@ -100,8 +100,8 @@ def copy_probe(gib: float) -> int:
 
 def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float, p_min: float, stage_passes: int = 2):
     """Time `steps` passes (whole passes, split across the engine's sub-pass streams as the library does by
-    default), then take the stage breakdown from `stage_passes` more passes with stage events (a pass with
-    stage events runs unsplit, on the engine's stream alone)."""
+    default), then take the stage breakdown from `stage_passes` more passes with stage events, after one not
+    counted (a pass with stage events runs whole, on the engine's stream alone)."""
     for _ in range(warmup):
         e.screen_async(form, chi2_max, 64, p_min)
         e.kept()          # sizes the selection buffers (a pass whose buffers overflowed is run again)
@@ -115,10 +115,11 @@ def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float,
     ms = (t1 - t0) * 1000.0 / steps
     e.set_timing(True)
     stage_sum: dict[str, float] = {}
-    for _ in range(stage_passes):
+    for rep in range(stage_passes + 1):           # the first pass with stage events is not counted
         e.screen_async(form, chi2_max, 64, p_min)
         for k, v in e.stage_times().items():     # waits for the pass; events are on the engine's stream
-            stage_sum[k] = stage_sum.get(k, 0.0) + v
+            if rep > 0:
+                stage_sum[k] = stage_sum.get(k, 0.0) + v
         e.kept()
     e.set_timing(False)
     return ms, {k: v / max(stage_passes, 1) for k, v in stage_sum.items()}

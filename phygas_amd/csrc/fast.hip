@@ -51,6 +51,7 @@ namespace phx {
 
 // ------------------------------------------------------------------------------------------- layout
 constexpr int SPW = 16;                 // stars per wavefront in the tile scan
+constexpr int TREC = 75;                // doubles per object tile of the tiled tracks (k_tiled)
 
 struct Cand { int star, cluster; unsigned long long mask; };
 struct Kept { int star, cluster, kb, pad; double sb, chi2; };
@@ -134,11 +135,12 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
 // Two lanes per star; lane q propagates offsets 3q .. 3q+2 (x, y, z for q = 0; vx, vy, vz for q = 1) and
 // both lanes run the same FP64 nominal.  Track layout as the plain path: [star][npts][12] FP64.
 __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
-                                                  double *__restrict__ trk, Pot P, PotF Q, int npts, int n) {
+                                                  double *__restrict__ trk, Pot P, PotF Q, int npts, int n,
+                                                  double *__restrict__ tiled, int ntl) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = gid >> 1, q = gid & 1;
     const bool live = s < n;
-    const double eps = PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
+    const double eps = PH_LIN_EPS_POS, inv_eps = 1.0 / PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
     V3 x = v3(0, 0, 0), v = v3(0, 0, 0);
     if (live) {
         const double *p = ps0 + (size_t)s * 6;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                const double mine = dx[i][r] / eps;
+                const double mine = dx[i][r] * inv_eps;
                 const double other = __shfl_xor(mine, 1, 64);
                 A[r][i] = q == 0 ? mine : other;
                 A[r][3 + i] = q == 0 ? other : mine;
@@ -220,6 +222,19 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
         out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = v.x; out[4] = v.y; out[5] = v.z;
 #pragma unroll
         for (int i = 0; i < 6; i++) out[6 + i] = S[i];
+        // the tiled copy (k_tiled's layout): point k in tile k / 8, and the far end of tile k / 8 - 1
+        const int j = k >> 3, p = k & 7;
+        double *tb = tiled + ((size_t)s * ntl + j) * TREC;
+        tb[p] = x.x; tb[9 + p] = x.y; tb[18 + p] = x.z;
+#pragma unroll
+        for (int f = 0; f < 6; f++) tb[27 + f * TILE + p] = S[f];
+        if (p == 0 && j > 0) { tb[8 - TREC] = x.x; tb[17 - TREC] = x.y; tb[26 - TREC] = x.z; }
+        if (k == npts - 1) {   // past the track: positions repeat the last one, covariances are 0
+            for (int pp = p + 1; pp <= TILE; pp++) { tb[pp] = x.x; tb[9 + pp] = x.y; tb[18 + pp] = x.z; }
+            for (int pp = p + 1; pp < TILE; pp++)
+#pragma unroll
+                for (int f = 0; f < 6; f++) tb[27 + f * TILE + pp] = 0.0;
+        }
     }
 }
 
@@ -508,7 +523,6 @@ __device__ __forceinline__ double point_chi2(V3 m0, V3 m1, bool has_next, const 
 // then the six covariance entries of the points 8j .. 8j+7 (8 each).  Eight lanes reading the same field of
 // eight consecutive points read 64 contiguous bytes.  Positions past the track repeat its last position and
 // covariances past it are 0 (never read for a pair: its points stop at the cluster's length).
-constexpr int TREC = 75;
 __global__ __launch_bounds__(256) void k_tiled(const double *__restrict__ trk, const int *__restrict__ cn, int n_const,
                                                int nobj, int npts, int ntl, double *__restrict__ out) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1283,12 +1297,12 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
         if (launch_lin_plain(ps0, sig, nullptr, trk, a.P, PH_LIN_EPS_POS, PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB,
                              PH_LIN_SUB, 0, npts, npts, ns, 0, nullptr, st))
             return 1;
-    } else {
+        hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts,
+                           ns, npts, ntl, (double *)W.mem[W_TILED]);
+    } else {   // writes the tiled copy too
         hipLaunchKernelGGL(k_lin_fast, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
-                           make_potf(a.P), npts, ns);
+                           make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
     }
-    hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts, ns,
-                       npts, ntl, (double *)W.mem[W_TILED]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
     hipLaunchKernelGGL(k_star_recs, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts, ntl, ns,

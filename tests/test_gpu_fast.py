@@ -175,6 +175,30 @@ def test_autotune_keeps_rows_and_stats_consistent(unit):
         _report(f"autotune_{k}_blocks", tune[f"{k}_blocks"])
 
 
+@pytest.mark.parametrize("ns", [0, 1, 17, 4100])
+def test_fast64_edge_batches_identical_to_plain(workload, ns):
+    """FAST64 rows equal PLAIN's byte for byte at the batch-size edges (empty, one star, a partial star block,
+    a split pass), on clusters that include 1- to 4-point tracks, with a tight and a wide chi2 cut."""
+    cl = workload.make_clusters(300, 0)
+    cl["tmax_myr"][:4] = [0.0, 0.5, 1.0, 1.5]          # tracks of 1, 2, 3 and 4 points
+    half = (ns + 1) // 2
+    st = workload.make_stars(max(half, 1), 2)[:half]
+    ctl = workload.shuffled_control(st, 3)[:ns - half] if ns > 1 else st[:0]
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        assert e.ns == len(st) + len(ctl)
+        for chi2_max, p_min in ((4.0, 0.0), (25.0, 0.0), (100.0, 0.001)):
+            plain = e.screen(hip.PLAIN, chi2_max=chi2_max, p_min=p_min)
+            kp = e.kept()
+            f64 = e.screen(hip.FAST64, chi2_max=chi2_max, p_min=p_min)
+            kf = e.kept()
+            assert kp == kf and plain.tobytes() == f64.tobytes(), (ns, chi2_max, p_min)
+            if ns == 0:
+                assert len(plain) == 0
+
+
 def test_split_passes_give_the_same_rows(workload):
     """A pass split into 2, 3 or 4 sub-passes on their own streams gives the whole pass's rows, work counts and
     tracks (also with hipGraph replays)."""

@@ -62,14 +62,13 @@ struct TileRec2 {   // FP64, level 2
 // level 1 rejects it against every cluster tile, with its squares finite in FP32.
 PHX_HDF TileRec1 pad_rec1() { return TileRec1{1e12f, 1e12f, 1e12f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}; }
 
-// Neighbouring floats (finite inputs): one step towards +inf / -inf.
+// The neighbouring float towards +inf (finite inputs).
 PHX_HDF float next_up(float f) {
     if (f == 0.0f) return 1.4e-45f;
     union { float f; int i; } u = {f};
     u.i += f > 0.0f ? 1 : -1;
     return u.f;
 }
-PHX_HDF float next_down(float f) { return -next_up(-f); }
 PHX_HDF float up(double v) {   // float >= v
     const float f = (float)v;
     return (double)f < v ? next_up(f) : f;

@@ -5,37 +5,38 @@
 //   1. draws         ps0 / Sigma0 per star (the plain FP64 kernel; 64 draws per star)
 //   2. k_lin_fast    tracks: FP64 nominal orbit, the six neighbour offsets of the finite-difference STM
 //                    propagated with the FP32 difference force (force32.h) and FP64 accumulators,
-//                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.
+//                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.  Writes the
+//                    AoS tracks and their tiled copy.
 //   3. k_star_recs   per (star, 8-point tile): the tile records of cull.h (axis line, tube radius and
-//                    Loewner covariance bound in the co-rotating frame)
+//                    Loewner covariance bound in the co-rotating frame); k_star_recs_coarse the coarse
+//                    (16-point) records from pairs of them
 //   4. k_scan_tiles  persistent, length-bucketed: each wavefront takes 64 clusters (sorted by track length)
-//                    x 16 stars, tests every (pair, tile) with the FP32 tube test and the survivors with the
-//                    FP64 Mahalanobis certificate (cull.h), and appends the pairs with any tile left, with a
-//                    bit mask of those tiles
+//                    x 16 stars, tests every (pair, coarse tile) with the FP32 tube test and the survivors
+//                    with the FP64 Mahalanobis certificate (cull.h), then the tiles of the coarse tiles left
+//                    with the certificate, and appends the pairs with any tile left, with a bit mask of those
+//                    tiles
 //   5. k_expand      one item per (candidate, tile left), contiguous per candidate
-//      k_items       one lane per item: the reference's FP64 evaluation of the tile's points (two
+//      k_items       eight lanes per item: the reference's FP64 evaluation of the tile's points (two
 //                    Mahalanobis forms per point, strict-< first-index minimum)
-//      k_reduce      eight lanes per candidate: minimum over its items; pairs with chi2 < chi2_max are
-//                    appended to the kept list
-//   6. k_finish_fast one wavefront per 8 kept pairs: the 512 Box-Muller draws of each pair's seed, screened
-//                    draw by draw (below) with the rare survivors evaluated through a wave-local queue;
-//                    exact counts; rows with p_r50 >= p_min appended
+//      k_reduce      eight lanes per candidate: minimum over its items; pairs with chi2 < chi2_max go to the
+//                    kept list (64 segments, packed by k_kept_compact)
+//   6. finish        k_fin_setup (frame and draw screen per kept pair), k_finish_fast (the 512 Box-Muller
+//                    draws of each pair's seed, screened draw by draw, below), k_fin_draw (the draws left,
+//                    one lane each), k_fin_rows / k_rows_rtot: exact counts; rows with p_r50 >= p_min
 //
-// Exactness of the cut: a tile dropped by the scan holds no point with chi2 < chi2_max (cull.h), and the
-// refine's point test drops a point only when chi2 >= d^2 / lambda_max(M_k) > chi2_max, with d the distance
-// of the step's segment from the origin (reference proof: src/phygas.c:360-369) and lambda_max(M_k) <=
-// lambda_s + lambda_c + R^2 (Weyl).  Kept pairs get the FP64 chi2, kb and sb of a full scan of the same
-// tracks.
+// Exactness of the cut: a tile dropped by the scan holds no point with chi2 < chi2_max (cull.h), so kept
+// pairs get the FP64 chi2, kb and sb of a full scan of the same tracks.
 //
 // Exactness of the finish.  splitmix64's state after i calls is seed + i*gamma, so draw i's six uniforms
-// are known by index.  x0 = m0 + l00 z0 needs |x0| < r to count, i.e. z0 in an interval I; z0 = rho cos(.)
-// with rho = sqrt(-2 ln u1), so a draw whose u1 exceeds exp(-dist(0, I)^2 / 2) cannot count and its other
-// five uniforms are never generated.  Otherwise z0 is estimated in FP32 (hardware log2 / cos); a draw whose
-// estimate is more than 1e-3 outside I cannot count.  For the rest all three z are estimated the same way,
-// and the draw is counted (or not) from the estimated position when it lies more than 1e-3 sum|L| from the
-// counting radius; only draws within that band are evaluated exactly (the reference's FP64 arithmetic).
-// The 1e-3 bound on |z estimate - z| holds with a factor ~40 to spare (phx_check_finish_screen measures it,
-// tail draws included).
+// are known by index.  A draw counts only when z = L^-1 (x - m) lies in the image of the counting ball, whose
+// projection on normal coordinate k is an interval I_k; z_k = rho cos(.) with rho = sqrt(-2 ln u) from the
+// draw's (2k+1)-th uniform u, so on the axis k whose interval lies farthest from 0 a draw whose u exceeds
+// exp(-dist(0, I_k)^2 / 2) cannot count, and its other uniforms are never generated.  Otherwise z_k is
+// estimated in FP32 (hardware log2 / cos); a draw whose estimate is more than 1e-3 outside I_k cannot count.
+// For the rest all three z are estimated the same way, and the draw is counted (or not) from the estimated
+// position when it lies more than 1e-3 sum|L| from the counting radius; only draws within that band are
+// evaluated exactly (the reference's FP64 arithmetic).  The 1e-3 bound on |z estimate - z| holds with a
+// factor ~40 to spare (phx_check_finish_screen measures it, tail draws included).
 #include <math.h>
 #include <stddef.h>
 #include <string.h>

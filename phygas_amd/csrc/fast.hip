@@ -338,6 +338,31 @@ __global__ __launch_bounds__(256) void k_cluster_recs_coarse(const double *__res
 //            cluster's lane as (star, tile) bits.
 // A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
 // drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
+// Level-2 records of a wavefront's 64 clusters in LDS, field-major ([field][lane], the 14 fields l2_pass
+// reads): the lanes' writes are consecutive and reads of different clusters fall in different banks, where a
+// [lane] array of 128-byte records puts every lane's copy of a field in one bank.
+constexpr int C2F = 14;
+__device__ __forceinline__ void lds_put_rec2(double (*d)[64], int i, const TileRec2 &r) {
+#pragma unroll
+    for (int u = 0; u < 3; u++) { d[u][i] = r.a[u]; d[3 + u][i] = r.b[u]; }
+#pragma unroll
+    for (int u = 0; u < 6; u++) d[6 + u][i] = r.U[u];
+    d[12][i] = r.e;
+    d[13][i] = r.R2;
+}
+__device__ __forceinline__ TileRec2 lds_get_rec2(const double (*d)[64], int i) {
+    TileRec2 r;
+#pragma unroll
+    for (int u = 0; u < 3; u++) { r.a[u] = d[u][i]; r.b[u] = d[3 + u][i]; }
+#pragma unroll
+    for (int u = 0; u < 6; u++) r.U[u] = d[6 + u][i];
+    r.e = d[12][i];
+    r.R2 = d[13][i];
+    r.t1 = 0.0;
+    r.pad = 0.0;
+    return r;
+}
+
 template <typename MaskT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
                                                     const TileRec2 *__restrict__ crec2c,
@@ -353,13 +378,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
     __shared__ unsigned short q_s[4][SPW * 64];
-    __shared__ TileRec2 c2_s[4][64];
+    __shared__ double c2_s[4][C2F][64];
     __shared__ float t1_s[4][64];
     __shared__ unsigned l2_s[4][64];
     __shared__ int nt_s[4][64], ncl_s[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned short *q = q_s[w];
-    TileRec2 *c2s = c2_s[w];
+    double (*c2s)[64] = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
     const int ntasks = ncb * nsb;
     // exclusive prefix of a per-lane count over the wavefront; the total into *tot
@@ -422,7 +447,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (nq == 0) continue;
             n_l2 += nq;
             enqueue(l1, o1);
-            if (l1) c2s[lane] = crec2c[(size_t)J * nc_pad + cs];
+            if (l1) lds_put_rec2(c2s, lane, crec2c[(size_t)J * nc_pad + cs]);
             t1_s[w][lane] = t1;
             l2_s[w][lane] = 0u;
             wave_sync();
@@ -431,7 +456,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (e < nq) {
                     const int ent = q[e], i = ent >> 6, c = ent & 63;
                     const TileRec2 s2 = srec2c[(size_t)(s0 + i) * ntlc + J];
-                    if (l2_pass<TILE2>(s2, c2s[c], (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
+                    if (l2_pass<TILE2>(s2, lds_get_rec2(c2s, c), (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
                 }
             }
             wave_sync();

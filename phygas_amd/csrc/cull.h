@@ -32,6 +32,11 @@
 //                   so min >= 2 (min(min_t y.(da + db t), min_t (R_1 y).(da + db t)) - rho |y|) - y^T U y;
 //                   y along U^-1 x* with x* the U^-1-closest point of the axis segment, scaled optimally,
 //                   makes the bound tight (l2_pass), compared with explicit bounds of its rounding.
+//   fine level 2:   the same dual bound with the tile's covariances interpolated in time (l2f_pass,
+//                   TileRecF): one U for all points inflates the narrow axes of an ellipsoid that turns
+//                   or shears over the tile, while (1 - w) F + w L + eta I, with the slack E of the
+//                   interpolation folded into F and L along the long axes, follows it; the bound becomes
+//                   a quadratic in the time parameter, checked over the tile without divisions.
 //
 // A tile that fails either test holds no point with chi2 < chi2_max (1 + CUT_REL), so it can hold neither
 // the minimum of a kept pair nor change which pairs are kept.  CUT_REL covers the reference's own FP64
@@ -57,6 +62,15 @@ struct TileRec1 {   // FP32, level 1
 struct TileRec2 {   // FP64, level 2
     double a[3], b[3], U[6], e, R2, t1, pad;
 };
+// Level 2 of the fine tiles, stored in FP32 (the test runs in FP64): the axis a, b and tube radius e, the
+// rotated covariances of the first and last points with an anisotropic slack E added to both (F, L; the
+// kernel term R^2 I included), and eta, the isotropic rest: every point k of the tile has its rotated
+// covariance below (1 - w) F + w L + eta I, w = (k - k0) / np, np = kp - k0.  The FP32 rounding of a and b
+// is part of e, and that of F and L part of eta (both rounded up), R2 is rounded down.  tp = kp - tc and
+// tl = kl - tc (last point and last position; tc = k0 + T / 2) are exact; ispan = 1 / np (0 when np = 0).
+struct TileRecF {
+    float a[3], b[3], F[6], L[6], e, eta, tp, tl, ispan, R2;
+};
 
 // Level-1 record of a star slot past the end of a batch (the scan reads whole star blocks): far enough that
 // level 1 rejects it against every cluster tile, with its squares finite in FP32.
@@ -73,6 +87,15 @@ PHX_HDF float up(double v) {   // float >= v
     const float f = (float)v;
     return (double)f < v ? next_up(f) : f;
 }
+PHX_HDF float down(double v) { return -up(-v); }   // float <= v
+
+// Gershgorin bound of |lambda| of a symmetric 3x3 (xx xy xz yy yz zz).
+PHX_HDF double sym3_abs_bound(const double *d) {
+    const double g0 = fabs(d[0]) + fabs(d[1]) + fabs(d[2]), g1 = fabs(d[1]) + fabs(d[3]) + fabs(d[4]),
+                 g2 = fabs(d[2]) + fabs(d[4]) + fabs(d[5]);
+    return fmax(g0, fmax(g1, g2));
+}
+
 
 // Largest-eigenvalue upper bound of a symmetric PSD 3x3 (xx xy xz yy yz zz): min(trace, Gershgorin).
 PHX_HDF double lam_bound(const double *S) {
@@ -223,21 +246,93 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
     kp_out = kp;
 }
 
-// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).
+// The interpolated record of a tile from the rotated covariances qs of its points k0 .. kp (qs[k - k0]), the
+// axis in r2 and trmax >= every trace.  D_k = q_k - ((1 - w_k) q_k0 + w_k q_kp) is the interpolation's error;
+// over a tile the covariance follows a smooth curve, so the D_k are nearly multiples of one matrix whose
+// positive part lies along the ellipsoid's long axes.  E = that positive part of the D_k with the largest
+// eigenvalue bound goes into F and L, and eta bounds what is left (E is approximate; eta is an upper bound).
+// (first = qs[0], last = qs[kp - k0]: passed apart so that every array index is a constant.)
 template <int T = TILE>
-PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2) {
+PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const double *last, int k0, int kp, int n,
+                           const TileRec2 &r2, double R2, double trmax, TileRecF &rf) {
+    const int kl = k0 + T < n - 1 ? k0 + T : n - 1;
+    const double tc = k0 + 0.5 * T;
+    const int np = kp - k0;
+    double D[T][6], Db[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, best = -1.0;
+#pragma unroll
+    for (int i = 0; i < T; i++) {
+        if (i <= np) {
+            const double w = np > 0 ? (double)i / (double)np : 0.0;
+#pragma unroll
+            for (int u = 0; u < 6; u++) D[i][u] = qs[i][u] - ((1.0 - w) * first[u] + w * last[u]);
+            const double lm = sym3_lmax_bound(D[i]);
+            if (lm > best) {
+                best = lm;
+#pragma unroll
+                for (int u = 0; u < 6; u++) Db[u] = D[i][u];
+            }
+        }
+    }
+    double E[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (best > 0.0) sym3_pos_part(Db, E);
+    double eta = 0.0;
+#pragma unroll
+    for (int i = 0; i < T; i++) {
+        if (i <= np) {
+            double d[6];
+#pragma unroll
+            for (int u = 0; u < 6; u++) d[u] = D[i][u] - E[u];
+            eta = fmax(eta, sym3_lmax_bound(d));
+        }
+    }
+    // FP32 storage: F - fl(F) and L - fl(L) are exact in FP64 and below their Gershgorin bounds times I
+    double dF[6], dL[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+        const double r = (u == 0 || u == 3 || u == 5) ? R2 : 0.0;
+        const double f = first[u] + E[u] + r, l = last[u] + E[u] + r;
+        rf.F[u] = (float)f;
+        rf.L[u] = (float)l;
+        dF[u] = f - (double)rf.F[u];
+        dL[u] = l - (double)rf.L[u];
+    }
+    // rounding of the rotations, the interpolation, the differences and the storage
+    rf.eta = up((eta * (1.0 + 1e-12) + 1e-13 * trmax + fmax(sym3_abs_bound(dF), sym3_abs_bound(dL))) * (1.0 + 1e-12));
+    // the stored axis is within |a - fl(a)| + (T / 2) |b - fl(b)| of the FP64 one over t in [-T/2, T/2]
+    double ea = 0.0, eb = 0.0;
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        rf.a[u] = (float)r2.a[u];
+        rf.b[u] = (float)r2.b[u];
+        const double xa = r2.a[u] - (double)rf.a[u], xb = r2.b[u] - (double)rf.b[u];
+        ea += xa * xa;
+        eb += xb * xb;
+    }
+    rf.e = up((r2.e + sqrt(ea) + 0.5 * T * sqrt(eb)) * (1.0 + 1e-12));
+    rf.tp = (float)(kp - tc);
+    rf.tl = (float)(kl - tc);
+    rf.ispan = np > 0 ? (float)(1.0 / (double)np) : 0.0f;
+    rf.R2 = down(R2);
+}
+
+// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).  rf (fine tiles
+// only): the interpolated record.
+template <int T = TILE>
+PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2,
+                          TileRecF *rf = nullptr) {
     int k0, kp;
     tile_line<T>(trk, n, j, r1, r2, k0, kp);
+    double qs[T][6];   // rotated covariances of the tile's points
     // Loewner bounds of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
     // and, for tiles of more than one point, sum + eta2 I (sum = the first point's + the last point's; over
     // a tile the covariance grows nearly quadratically in time, so every point's lies close to or below the
     // chord between the ends, and eta2 is usually 0 even where the ellipsoid shears and eta1 is as large as
     // its narrow axes).  Level 2 takes the one with the larger tr(U^-1); level 1 the smaller eigenvalue bound.
-    double ref[6], sum[6];
+    double ref[6], sum[6], first[6];
     {
         const double th = OMEGA_PT * kp, t0 = OMEGA_PT * k0;
         rot_cov(trk + (size_t)kp * 12 + 6, cos(th), sin(th), ref);
-        double first[6], d[6], pp[6];
+        double d[6], pp[6];
         rot_cov(trk + (size_t)k0 * 12 + 6, cos(t0), sin(t0), first);
 #pragma unroll
         for (int u = 0; u < 6; u++) d[u] = first[u] - ref[u];
@@ -251,7 +346,7 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
         const int k = k0 + i;
         if (k <= kp) {
             const double th = OMEGA_PT * k;
-            double q[6];
+            double *q = qs[i];
             rot_cov(trk + (size_t)k * 12 + 6, cos(th), sin(th), q);
             double d1[6], d2[6];
 #pragma unroll
@@ -273,6 +368,7 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
 #pragma unroll
     for (int u = 0; u < 6; u++) r2.U[u] = U[u];
     r2.R2 = R2;
+    if (rf) tile_record_f<T>(qs, first, ref, k0, kp, n, r2, R2, trmax, *rf);
 }
 
 // Records of the coarse tile J (TILE2 = 2 TILE points) from the records fa, fb of its tiles 2J, 2J + 1 (fb null
@@ -395,6 +491,95 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     const double err_u = 1e-12 * tr * nz2;
     const double L = lin - rho * nz - err_l;
     return !(L > 0.0 && L * L > chi2c * (zuz + err_u) * (1.0 + 1e-12));
+}
+
+// FP32 check that g(t) = l(t)^2 - c V(t) > 0 and l(t) > 0 over [ta, tb] for the true l(t) >= l0 + l1 t - el
+// and V(t) <= v0 + v1 t + ev, el = 1e-6 (|l0| + |l1| tm) and ev = 1e-6 (|v0| + |v1| tm) (tm >= |t|) covering
+// the rounding of l0, l1, v0, v1 and of their evaluation: l is linear, so its lower bound positive at both
+// ends suffices; g is a quadratic, so its ends and, when it lies inside, its vertex (4 A C > B^2, division
+// free), each against a margin well above the FP32 rounding of the comparison.
+PHX_HDF bool quad_clear(float l0, float l1, float v0, float v1, float c, float ta, float tb, float tm) {
+    const float el = 1e-6f * f_fma(fabsf(l1), tm, fabsf(l0)), ev = 1e-6f * f_fma(fabsf(v1), tm, fabsf(v0));
+    const float la = f_fma(l1, ta, l0) - el, lb = f_fma(l1, tb, l0) - el;
+    if (!(la > 0.0f) || !(lb > 0.0f)) return false;
+    const float va = c * (f_fma(v1, ta, v0) + ev), vb = c * (f_fma(v1, tb, v0) + ev);
+    if (!(f_fma(la, la, -va) > 1e-5f * f_fma(la, la, va))) return false;
+    if (!(f_fma(lb, lb, -vb) > 1e-5f * f_fma(lb, lb, vb))) return false;
+    const float A = l1 * l1, B = f_fma(2.0f * l0, l1, -c * v1), C = f_fma(l0, l0, -c * v0);
+    if (A > 0.0f && -B > 2.0f * A * ta && -B < 2.0f * A * tb) {   // the vertex -B / 2A inside
+        // its value C - B^2 / 4A moves by at most 2 |l| el + c ev under the perturbations; 4A times that and
+        // the rounding of the discriminant are below the margin
+        const float lm = f_fma(fabsf(l1), tm, fabsf(l0)), vm = f_fma(fabsf(v1), tm, fabsf(v0));
+        const float m = 1e-4f * f_fma(4.0f * A, f_fma(lm, lm, c * vm), B * B);
+        return f_fma(4.0f * A, C, -B * B) > m;
+    }
+    return true;
+}
+
+// Fine level 2 with the covariances interpolated in time (TileRecF), in FP32 with explicit error bounds.
+// Segment k of the pair (k0 <= k <= kp, kp the cluster's, which ends no later than the star's) has its near
+// end at time parameter tau_k = k - tc and its far end (the next position, seen from frame k through R_1) at
+// tau_k + 1; every point x of it has z.x >= min(l_a(tau_k), l_b(tau_k + 1)), l_a(t) = z.(da + db t) - rho |z|
+// and l_b the same with R_1 z, and its covariance M_k has z^T M_k z <= V(tau_k), V(t) = v0 + v1 t the sum of
+// the two objects' interpolations (each linear in t between its own first and last points) plus eta |z|^2.
+// With z.x > 0, q >= (z.x)^2 / z^T M_k z, so the tile is dropped when l_a(t)^2 > chi2c V(t) over [t0, tp]
+// and l_b(t)^2 > chi2c V(t - 1) over [t0 + 1, tl].  z = U^-1 x* as in l2_pass with U the mean of the
+// interpolation's ends, normalised (|z| = 1 to ~1e-7), so that every term is of the size of the geometry.
+// Rounding: the differences da, db, the dot products and the quadratic forms of the PSD F, L are within
+// ~1e-6 relative of their absolute-value bounds, far below the 1e-5 taken for them.
+template <int T = TILE>
+PHX_HDF bool l2f_pass(const TileRecF &s, const TileRecF &c, float chi2c) {
+    const float da0 = s.a[0] - c.a[0], da1 = s.a[1] - c.a[1], da2 = s.a[2] - c.a[2];
+    const float db0 = s.b[0] - c.b[0], db1 = s.b[1] - c.b[1], db2 = s.b[2] - c.b[2];
+    const float eta = s.eta + c.eta;
+    float U[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) U[u] = 0.5f * ((s.F[u] + s.L[u]) + (c.F[u] + c.L[u]));
+    const float trF = (s.F[0] + s.F[3] + s.F[5]) + (c.F[0] + c.F[3] + c.F[5]);
+    const float trL = (s.L[0] + s.L[3] + s.L[5]) + (c.L[0] + c.L[3] + c.L[5]);
+    // every M_k has trace <= max(trF, trL) + 3 eta and smallest eigenvalue >= R^2: condition number bounded
+    // (the bound taken 1% below 1e9 for the rounding)
+    if (!(f_fma(3.0f, eta, fmaxf(trF, trL)) < 0.99e9f * (s.R2 + c.R2))) return true;
+    U[0] += eta; U[3] += eta; U[5] += eta;
+    const float t0 = -0.5f * T, tp = fminf(s.tp, c.tp), tl = fminf(s.tl, c.tl), tm = 0.5f * T;
+    auto dot = [](float a0, float a1, float a2, float b0, float b1, float b2) {
+        return f_fma(a0, b0, f_fma(a1, b1, a2 * b2));
+    };
+    // z = U^-1 x* through the Cholesky factor U = L L^T (backward stable in FP32, where an adjugate of an
+    // elongated U is not): w = L^-1 (da + db t), t* minimising |w|^2 over [t0, tl], z = L^-T w(t*)
+    const float i0 = f_rsqrt(U[0]), l10 = U[1] * i0, l20 = U[2] * i0;
+    const float i1 = f_rsqrt(f_fma(-l10, l10, U[3])), l21 = f_fma(-l20, l10, U[4]) * i1;
+    const float i2 = f_rsqrt(f_fma(-l21, l21, f_fma(-l20, l20, U[5])));
+    const float wa0 = da0 * i0, wa1 = f_fma(-l10, wa0, da1) * i1, wa2 = f_fma(-l21, wa1, f_fma(-l20, wa0, da2)) * i2;
+    const float wb0 = db0 * i0, wb1 = f_fma(-l10, wb0, db1) * i1, wb2 = f_fma(-l21, wb1, f_fma(-l20, wb0, db2)) * i2;
+    const float B = dot(wa0, wa1, wa2, wb0, wb1, wb2), C = dot(wb0, wb1, wb2, wb0, wb1, wb2);
+    float ts = C > 0.0f ? -B * f_rcp(C) : t0;
+    ts = fminf(fmaxf(ts, t0), tl);
+    const float x0 = f_fma(wb0, ts, wa0), x1 = f_fma(wb1, ts, wa1), x2 = f_fma(wb2, ts, wa2);
+    float z2 = x2 * i2, z1 = f_fma(-l21, z2, x1) * i1, z0 = f_fma(-l20, z2, f_fma(-l10, z1, x0)) * i0;
+    const float n2 = dot(z0, z1, z2, z0, z1, z2);
+    if (!(n2 > 1e-30f && n2 < 1e30f)) return true;
+    const float r = f_rsqrt(n2);
+    z0 *= r; z1 *= r; z2 *= r;   // |z| <= 1 + 1e-6
+    const float zr0 = f_fma((float)ROT_C, z0, -(float)ROT_S * z1), zr1 = f_fma((float)ROT_S, z0, (float)ROT_C * z1);
+    const float p00 = z0 * z0, p11 = z1 * z1, p22 = z2 * z2, p01 = 2.0f * z0 * z1, p02 = 2.0f * z0 * z2,
+                p12 = 2.0f * z1 * z2;
+    auto qf = [&](const float *M) {
+        return f_fma(p00, M[0], f_fma(p11, M[3], f_fma(p22, M[5], f_fma(p01, M[1], f_fma(p02, M[2], p12 * M[4])))));
+    };
+    const float VFs = qf(s.F), VLs = qf(s.L), VFc = qf(c.F), VLc = qf(c.L);
+    const float v1 = f_fma(VLs - VFs, s.ispan, (VLc - VFc) * c.ispan);
+    // |z|^2 <= 1 + 3e-6; for a PSD M, |M_ij| <= sqrt(M_ii M_jj), so sum |p_ij M_ij| <= tr(M) |z|^2 and each
+    // form is within 7 u tr(M) of its value; ispan is within u of 1 / np
+    const float v0 = f_fma(-v1, t0, VFs + VFc) + f_fma(eta, 1.00001f, 2e-6f * (trF + trL + eta));
+    const float rho = (s.e + c.e) * 1.00001f;
+    const float na1 = fabsf(da0) + fabsf(da1) + fabsf(da2), nb1 = fabsf(db0) + fabsf(db1) + fabsf(db2);
+    const float el = 1e-5f * (f_fma(tm, nb1, na1) + rho);   // the differences' and dot products' rounding
+    const float la0 = dot(z0, z1, z2, da0, da1, da2) - (rho + el), la1 = dot(z0, z1, z2, db0, db1, db2);
+    const float lb0 = dot(zr0, zr1, z2, da0, da1, da2) - (rho + el), lb1 = dot(zr0, zr1, z2, db0, db1, db2);
+    const bool clear = quad_clear(la0, la1, v0, v1, chi2c, t0, tp, tm) &&
+                       (tl < t0 + 1.0f || quad_clear(lb0, lb1, v0 - v1, v1, chi2c, t0 + 1.0f, tl, tm));   // no far end: kl = k0
+    return !clear;
 }
 
 }  // namespace phx

@@ -8,12 +8,13 @@
 //                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.  Writes the
 //                    AoS tracks and their tiled copy.
 //   3. k_star_recs   per (star, 8-point tile): the tile records of cull.h (axis line, tube radius and
-//                    Loewner covariance bound in the co-rotating frame); k_star_recs_coarse the coarse
+//                    Loewner covariance bounds in the co-rotating frame: one bound for the tile, and the
+//                    interpolation between its ends for the fine level); k_star_recs_coarse the coarse
 //                    (16-point) records from pairs of them
 //   4. k_scan_tiles  persistent, length-bucketed: each wavefront takes 64 clusters (sorted by track length)
 //                    x 16 stars, tests every (pair, coarse tile) with the FP32 tube test and the survivors
 //                    with the FP64 Mahalanobis certificate (cull.h), then the tiles of the coarse tiles left
-//                    with the certificate, and appends the pairs with any tile left, with a bit mask of those
+//                    with the interpolated certificate, and appends the pairs with any tile left, with a bit mask of those
 //                    tiles
 //   5. k_expand      one item per (candidate, tile left), contiguous per candidate
 //      k_items       eight lanes per item: the reference's FP64 evaluation of the tile's points (two
@@ -59,7 +60,8 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
-       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SREC2C, W_NSLOT };
+       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SREC2C, W_SRECF, W_NSLOT };
+static_assert(W_NSLOT <= 24, "FastWork holds 24 buffers");
 // Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
 // for the exact evaluation by the finish screen, largest kept segment, level-1 tests of the scan
 enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
@@ -68,7 +70,8 @@ enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_L1, C
 // atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
 constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32, C_TOTAL = C_KSEG0 + KSEG * KSEG_STRIDE;
 // Table buffers (FastTables::mem slots)
-enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CREC2C, T_NSLOT };
+enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CREC2C, T_CRECF, T_NSLOT };
+static_assert(T_NSLOT <= 16, "FastTables holds 16 buffers");
 
 static int ensure(void *&p, size_t &have, size_t need) {
     if (need <= have && p) return 0;
@@ -240,18 +243,21 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
 }
 
 // ------------------------------------------------------------------------------------------- tile records
-// One lane per (star, tile): records of the star tracks [ns][npts][12] into srec1 / srec2 [ns_pad][ntl]; rows
-// past the last star hold records no pair test accepts (the scan reads whole blocks of SPW stars).
+// One lane per (star, tile): records of the star tracks [ns][npts][12] into srec1 / srec2 / srecf [ns_pad][ntl];
+// rows past the last star hold records no pair test accepts (the scan reads whole blocks of SPW stars).
 __global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ trk, int npts, int ntl, int n, int n_pad,
-                                                   TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2) {
+                                                   TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2,
+                                                   TileRecF *__restrict__ rf) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)n_pad * ntl) return;
     const int s = (int)(id / ntl), j = (int)(id % ntl);
     TileRec1 a = pad_rec1();
     TileRec2 b = {};
-    if (s < n) tile_records(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b);
+    TileRecF f = {};
+    if (s < n) tile_records(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b, &f);
     r1[id] = a;
     r2[id] = b;
+    rf[id] = f;
 }
 
 // Coarse records (TILE2 points: tiles 2J, 2J + 1) of the stars from their tile records r2 [n_pad][ntl]:
@@ -291,18 +297,21 @@ __global__ __launch_bounds__(256) void k_cluster_aos(const double *__restrict__ 
 __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__ ctrk, const int *__restrict__ cn,
                                                       const double *__restrict__ R2, const int *__restrict__ perm,
                                                       int nc, int nc_pad, int npts, int ntl,
-                                                      TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2) {
+                                                      TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2,
+                                                      TileRecF *__restrict__ rf) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)nc_pad * ntl) return;
     const int j = (int)(id / nc_pad), cs = (int)(id % nc_pad);
     TileRec1 a = {};
     TileRec2 b = {};
+    TileRecF f = {};
     if (cs < nc) {
         const int c = perm[cs], n = cn[c];
-        if (j * TILE < n) tile_records(ctrk + (size_t)c * npts * 12, n, j, R2[c], a, b);
+        if (j * TILE < n) tile_records(ctrk + (size_t)c * npts * 12, n, j, R2[c], a, b, &f);
     }
     r1[id] = a;
     r2[id] = b;
+    rf[id] = f;
 }
 
 // Coarse records of the clusters from their tile records r2 [ntl][nc_pad] (sorted order): rc1 / rc2
@@ -333,8 +342,9 @@ __global__ __launch_bounds__(256) void k_cluster_recs_coarse(const double *__res
 //            is wave-uniform and read through scalar loads);
 //   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
 //            evaluated 64 at a time, one per lane, with the cluster's coarse records staged in LDS;
-//   fine     the coarse tiles left, queued again, each entry testing tiles 2J and 2J + 1 with level 2 on
-//            their own records (level 1 would pass nearly all of them).  The verdicts return to the
+//   fine     the coarse tiles left, queued again, each entry testing tiles 2J and 2J + 1 with the fine
+//            level 2 (l2f_pass, covariances interpolated over the tile) on their own records (level 1
+//            would pass nearly all of them).  The verdicts return to the
 //            cluster's lane as (star, tile) bits.
 // A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
 // drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
@@ -366,12 +376,12 @@ __device__ __forceinline__ TileRec2 lds_get_rec2(const double (*d)[64], int i) {
 template <typename MaskT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
                                                     const TileRec2 *__restrict__ crec2c,
-                                                    const TileRec2 *__restrict__ crec2,
+                                                    const TileRecF *__restrict__ crecf,
                                                     const int *__restrict__ ntc_sorted,
                                                     const int *__restrict__ cn_sorted, const int *__restrict__ perm,
                                                     int nc, int nc_pad, const TileRec1 *__restrict__ srec1c,
                                                     const TileRec2 *__restrict__ srec2c,
-                                                    const TileRec2 *__restrict__ srec2, int ns, int ntl, int ntlc,
+                                                    const TileRecF *__restrict__ srecf, int ns, int ntl, int ntlc,
                                                     float kchi, double chi2c, Cand *__restrict__ cand,
                                                     unsigned long long *__restrict__ ncand, long cap,
                                                     unsigned long long *__restrict__ next_task,
@@ -383,6 +393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ unsigned l2_s[4][64];
     __shared__ int nt_s[4][64], ncl_s[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const float chi2cf = up(chi2c);   // the fine level's FP32 cut, rounded up
     unsigned short *q = q_s[w];
     double (*c2s)[64] = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
@@ -479,9 +490,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         const int j = 2 * J + h;
                         if (j < ntc) {
                             fine_l2++;
-                            const TileRec2 s2 = srec2[(size_t)(s0 + i) * ntl + j];
-                            const TileRec2 c2 = crec2[(size_t)j * nc_pad + csc];
-                            if (l2_pass(s2, c2, (double)tile_t1(nclc, j), chi2c)) atomicOr(&l2_s[w][c], 1u << (2 * i + h));
+                            if (l2f_pass(srecf[(size_t)(s0 + i) * ntl + j], crecf[(size_t)j * nc_pad + csc], chi2cf))
+                                atomicOr(&l2_s[w][c], 1u << (2 * i + h));
                         }
                     }
                 }
@@ -1253,7 +1263,8 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
         ensure(T.mem[T_CREC1], T.bytes[T_CREC1], sizeof(TileRec1) * nrec) ||
         ensure(T.mem[T_CREC2], T.bytes[T_CREC2], sizeof(TileRec2) * nrec) ||
         ensure(T.mem[T_CREC1C], T.bytes[T_CREC1C], sizeof(TileRec1) * nrec_c) ||
-        ensure(T.mem[T_CREC2C], T.bytes[T_CREC2C], sizeof(TileRec2) * nrec_c))
+        ensure(T.mem[T_CREC2C], T.bytes[T_CREC2C], sizeof(TileRec2) * nrec_c) ||
+        ensure(T.mem[T_CRECF], T.bytes[T_CRECF], sizeof(TileRecF) * nrec))
         return 1;
     nt_sorted.resize(T.nc_pad, 0);
     cn_sorted.resize(T.nc_pad, 0);
@@ -1263,7 +1274,7 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
     PHX_CK(hipMemcpyAsync(T.mem[T_CPERM], perm.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_cluster_recs, dim3(blocks_for((long)nrec, 256)), dim3(256), 0, st,
                        (const double *)T.mem[T_CTRK], d_cn, d_R2, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
-                       T.ntiles, (TileRec1 *)T.mem[T_CREC1], (TileRec2 *)T.mem[T_CREC2]);
+                       T.ntiles, (TileRec1 *)T.mem[T_CREC1], (TileRec2 *)T.mem[T_CREC2], (TileRecF *)T.mem[T_CRECF]);
     hipLaunchKernelGGL(k_cluster_recs_coarse, dim3(blocks_for((long)nrec_c, 256)), dim3(256), 0, st,
                        (const double *)T.mem[T_CTRK], d_cn, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
                        n_coarse(T.ntiles), (const TileRec2 *)T.mem[T_CREC2], (TileRec1 *)T.mem[T_CREC1C],
@@ -1291,6 +1302,7 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
         ensure(W.mem[W_SREC2], W.bytes[W_SREC2], sizeof(TileRec2) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_SREC1C], W.bytes[W_SREC1C], sizeof(TileRec1) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
         ensure(W.mem[W_SREC2C], W.bytes[W_SREC2C], sizeof(TileRec2) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
+        ensure(W.mem[W_SRECF], W.bytes[W_SRECF], sizeof(TileRecF) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_CAND], W.bytes[W_CAND], sizeof(Cand) * (size_t)W.cand_cap) ||
         ensure(W.mem[W_KEPT], W.bytes[W_KEPT], sizeof(Kept) * (size_t)W.kept_cap) ||
         ensure(W.mem[W_ROWS], W.bytes[W_ROWS], sizeof(PhxRow) * (size_t)W.row_cap) ||
@@ -1332,7 +1344,7 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
     hipLaunchKernelGGL(k_star_recs, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts, ntl, ns,
-                       ns_pad(ns), srec1, srec2);
+                       ns_pad(ns), srec1, srec2, (TileRecF *)W.mem[W_SRECF]);
     hipLaunchKernelGGL(k_star_recs_coarse, dim3(blocks_for((long)ns_pad(ns) * n_coarse(ntl), 256)), dim3(256), 0, st,
                        trk, npts, ntl, n_coarse(ntl), ns, ns_pad(ns), (const TileRec2 *)srec2,
                        (TileRec1 *)W.mem[W_SREC1C], (TileRec2 *)W.mem[W_SREC2C]);
@@ -1342,9 +1354,10 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
     hipLaunchKernelGGL(scan, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
                        (const TileRec1 *)T.mem[T_CREC1C], (const TileRec2 *)T.mem[T_CREC2C],
-                       (const TileRec2 *)T.mem[T_CREC2], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
+                       (const TileRecF *)T.mem[T_CRECF], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
-                       (const TileRec2 *)W.mem[W_SREC2C], srec2, ns, ntl, n_coarse(ntl), up(sqrt(chi2c)), chi2c,
+                       (const TileRec2 *)W.mem[W_SREC2C], (const TileRecF *)W.mem[W_SRECF], ns, ntl, n_coarse(ntl),
+                       up(sqrt(chi2c)), chi2c,
                        (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK, cnt + C_L1, cnt + C_L2);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "scan");

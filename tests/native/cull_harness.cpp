@@ -5,14 +5,19 @@
 
 using namespace phx;
 
+// p2: level 2 of the fine tiles as the scan runs it (l2f_pass); *n_l2u: level-1 survivors that the single-U
+// level 2 (l2_pass) would leave.
 extern "C" int cc_scan(const double *strk, int ns, int snpts, const double *ctrk, const int *cn, const double *R2,
-                       int nc, int cnpts, double chi2_max, unsigned char *p1, unsigned char *p2) {
+                       int nc, int cnpts, double chi2_max, unsigned char *p1, unsigned char *p2, long *n_l2u) {
     const int ntl = (snpts + TILE - 1) / TILE;
     TileRec1 *s1 = new TileRec1[(size_t)ns * ntl];
     TileRec2 *s2 = new TileRec2[(size_t)ns * ntl];
+    TileRecF *sf = new TileRecF[(size_t)ns * ntl];
     for (int s = 0; s < ns; s++)
         for (int j = 0; j < ntl; j++)
-            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j]);
+            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j],
+                         &sf[(size_t)s * ntl + j]);
+    *n_l2u = 0;
     const double chi2c = chi2_max * (1.0 + CUT_REL);
     const float kchi = up(sqrt(chi2c));
     for (int c = 0; c < nc; c++) {
@@ -20,18 +25,21 @@ extern "C" int cc_scan(const double *strk, int ns, int snpts, const double *ctrk
         for (int j = 0; j < ntl; j++) {
             TileRec1 c1 = {};
             TileRec2 c2 = {};
-            if (j < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], c1, c2);
+            TileRecF cf = {};
+            if (j < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], c1, c2, &cf);
             const float t1 = tile_t1(n, j);
             for (int s = 0; s < ns; s++) {
                 const size_t o = ((size_t)s * nc + c) * ntl + j;
                 const bool a = j < nt && l1_pass(s1[(size_t)s * ntl + j], c1, t1, kchi);
                 p1[o] = a;
-                p2[o] = a && l2_pass(s2[(size_t)s * ntl + j], c2, (double)t1, chi2c);
+                p2[o] = a && l2f_pass(sf[(size_t)s * ntl + j], cf, up(chi2c));
+                *n_l2u += a && l2_pass(s2[(size_t)s * ntl + j], c2, (double)t1, chi2c);
             }
         }
     }
     delete[] s1;
     delete[] s2;
+    delete[] sf;
     return ntl;
 }
 
@@ -65,9 +73,11 @@ extern "C" int cc_scan2(const double *strk, int ns, int snpts, const double *ctr
     const int ntl = (snpts + TILE - 1) / TILE, ntl2 = (snpts + TILE2 - 1) / TILE2;
     TileRec1 *s1 = new TileRec1[(size_t)ns * ntl], *q1 = new TileRec1[(size_t)ns * ntl2];
     TileRec2 *s2 = new TileRec2[(size_t)ns * ntl], *q2 = new TileRec2[(size_t)ns * ntl2];
+    TileRecF *sf = new TileRecF[(size_t)ns * ntl];
     for (int s = 0; s < ns; s++) {
         for (int j = 0; j < ntl; j++)
-            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j]);
+            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j],
+                         &sf[(size_t)s * ntl + j]);
         for (int J = 0; J < ntl2; J++) {
             const TileRec2 *f = s2 + (size_t)s * ntl + 2 * J;
             coarse_records(strk + (size_t)s * snpts * 12, snpts, J, f[0], 2 * J + 1 < ntl ? f + 1 : nullptr,
@@ -82,8 +92,9 @@ extern "C" int cc_scan2(const double *strk, int ns, int snpts, const double *ctr
         for (int J = 0; J < ntl2; J++) {
             TileRec1 c1 = {}, d1[2] = {};
             TileRec2 c2 = {}, d2[2] = {};
+            TileRecF df[2] = {};
             for (int h = 0; h < 2; h++)
-                if (2 * J + h < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, 2 * J + h, R2[c], d1[h], d2[h]);
+                if (2 * J + h < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, 2 * J + h, R2[c], d1[h], d2[h], &df[h]);
             if (J < nt2) coarse_records(ctrk + (size_t)c * cnpts * 12, n, J, d2[0], 2 * J + 1 < nt ? &d2[1] : nullptr, c1, c2);
             for (int s = 0; s < ns; s++) {
                 for (int h = 0; h < 2; h++)
@@ -99,11 +110,11 @@ extern "C" int cc_scan2(const double *strk, int ns, int snpts, const double *ctr
                     counts[0]++;
                     if (!l1_pass(s1[(size_t)s * ntl + j], d1[h], tile_t1(n, j), kchi)) continue;
                     counts[1]++;
-                    p2[((size_t)s * nc + c) * ntl + j] = l2_pass(s2[(size_t)s * ntl + j], d2[h], (double)tile_t1(n, j), chi2c);
+                    p2[((size_t)s * nc + c) * ntl + j] = l2f_pass(sf[(size_t)s * ntl + j], df[h], up(chi2c));
                 }
             }
         }
     }
-    delete[] s1; delete[] s2; delete[] q1; delete[] q2;
+    delete[] s1; delete[] s2; delete[] q1; delete[] q2; delete[] sf;
     return ntl;
 }

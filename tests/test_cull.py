@@ -27,7 +27,7 @@ def harness(tmp_path_factory):
     lib = C.CDLL(out)
     lib.cc_scan.restype = C.c_int
     P, I, D = C.c_void_p, C.c_int, C.c_double
-    lib.cc_scan.argtypes = [P, I, I, P, P, P, I, I, D, P, P]
+    lib.cc_scan.argtypes = [P, I, I, P, P, P, I, I, D, P, P, P]
     return lib
 
 
@@ -84,8 +84,9 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     ntl = (npts + 7) // 8
     p1 = np.zeros((len(st), len(cl), ntl), np.uint8)
     p2 = np.zeros_like(p1)
+    n_l2u = np.zeros(1, np.int64)
     got = harness.cc_scan(stt.ctypes.data, len(st), npts, ct.ctypes.data, cn.ctypes.data, R2.ctypes.data, len(cl),
-                          npts, CHI2, p1.ctypes.data, p2.ctypes.data)
+                          npts, CHI2, p1.ctypes.data, p2.ctypes.data, n_l2u.ctypes.data)
     assert got == ntl
     q = _point_chi2(stt, ct, cn, R2)
     qt = np.full(p1.shape, np.inf)
@@ -107,6 +108,8 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     f1 = p1[np.broadcast_to(exists, p1.shape)].mean()
     f2 = p2[np.broadcast_to(exists, p2.shape)].mean()
     assert f1 < 0.35 and f2 < 0.12, (f1, f2)
+    # the interpolated covariances leave fewer tiles than one bound for the whole tile
+    assert p2.sum() < 0.85 * n_l2u[0], (p2.sum(), n_l2u[0])
     # the scan's order: coarse tiles (two tiles) through both levels first, then the tiles of those left
     harness.cc_scan2.restype = C.c_int
     harness.cc_scan2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

@@ -140,6 +140,20 @@ PHX_HDF void rot_cov(const double *S, double c, double s, double *o) {
     o[5] = S[5];
 }
 
+// cos and sin of the frame rotation at point k (OMEGA_PT k): from the table cs ([k][2]) when given, else
+// computed.  Only the consistency of one point's rotation of its position and its covariance matters to a
+// record (chi2 is invariant under any invertible map applied to both).
+PHX_HDF void rot_at(const double *cs, int k, double &c, double &s) {
+    if (cs) {
+        c = cs[2 * k];
+        s = cs[2 * k + 1];
+    } else {
+        const double th = OMEGA_PT * k;
+        c = cos(th);
+        s = sin(th);
+    }
+}
+
 // Positive part of a symmetric 3x3 (xx xy xz yy yz zz): V diag(max(lambda, 0)) V^T from cyclic Jacobi
 // rotations.  Approximate (the callers bound what it misses).
 PHX_HDF void sym3_pos_part(const double *D, double *P) {
@@ -190,7 +204,8 @@ PHX_HDF double inv_trace(const double *S) {
 // Axis of tile j (T points) of an object with n points: the line fit and tube radius fields of r1 / r2 and
 // t1; returns the first and last point indices (k0, kp).  trk = its FP64 track [n][12] (pos, vel, S).
 template <int T = TILE>
-PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &r2, int &k0_out, int &kp_out) {
+PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &r2, int &k0_out, int &kp_out,
+                       const double *cs = nullptr) {
     const int k0 = j * T;
     const int kl = k0 + T < n - 1 ? k0 + T : n - 1;   // last position
     const int kp = (k0 + T < n ? k0 + T : n) - 1;     // last point
@@ -202,7 +217,8 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
     for (int i = 0; i <= T; i++) {
         const int k = k0 + (i < m ? i : m - 1);
         const double *p = trk + (size_t)k * 12;
-        const double th = OMEGA_PT * k, c = cos(th), s = sin(th);
+        double c, s;
+        rot_at(cs, k, c, s);
         px[i] = c * p[0] - s * p[1];
         py[i] = s * p[0] + c * p[1];
         pz[i] = p[2];
@@ -254,7 +270,7 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
 // (first = qs[0], last = qs[kp - k0]: passed apart so that every array index is a constant.)
 template <int T = TILE>
 PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const double *last, int k0, int kp, int n,
-                           const TileRec2 &r2, double R2, double trmax, TileRecF &rf) {
+                           const TileRec2 &r2, double R2, double trmax, TileRecF &rf, TileRec1 &r1) {
     const int kl = k0 + T < n - 1 ? k0 + T : n - 1;
     const double tc = k0 + 0.5 * T;
     const int np = kp - k0;
@@ -286,18 +302,24 @@ PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const dou
         }
     }
     // FP32 storage: F - fl(F) and L - fl(L) are exact in FP64 and below their Gershgorin bounds times I
-    double dF[6], dL[6];
+    double F[6], L[6], dF[6], dL[6];
 #pragma unroll
     for (int u = 0; u < 6; u++) {
         const double r = (u == 0 || u == 3 || u == 5) ? R2 : 0.0;
-        const double f = first[u] + E[u] + r, l = last[u] + E[u] + r;
-        rf.F[u] = (float)f;
-        rf.L[u] = (float)l;
-        dF[u] = f - (double)rf.F[u];
-        dL[u] = l - (double)rf.L[u];
+        F[u] = first[u] + E[u] + r;
+        L[u] = last[u] + E[u] + r;
+        rf.F[u] = (float)F[u];
+        rf.L[u] = (float)L[u];
+        dF[u] = F[u] - (double)rf.F[u];
+        dL[u] = L[u] - (double)rf.L[u];
     }
-    // rounding of the rotations, the interpolation, the differences and the storage
-    rf.eta = up((eta * (1.0 + 1e-12) + 1e-13 * trmax + fmax(sym3_abs_bound(dF), sym3_abs_bound(dL))) * (1.0 + 1e-12));
+    // rounding of the rotations, the interpolation and the differences
+    eta = eta * (1.0 + 1e-12) + 1e-13 * trmax;
+    // level 1: every M_k is below (1 - w) F + w L + eta I, whose largest eigenvalue is at most the larger of
+    // F's and L's plus eta
+    r1.lam = up((fmax(lam_bound(F), lam_bound(L)) + eta) * (1.0 + 1e-6));
+    // and the storage
+    rf.eta = up((eta + fmax(sym3_abs_bound(dF), sym3_abs_bound(dL))) * (1.0 + 1e-12));
     // the stored axis is within |a - fl(a)| + (T / 2) |b - fl(b)| of the FP64 one over t in [-T/2, T/2]
     double ea = 0.0, eb = 0.0;
 #pragma unroll
@@ -315,48 +337,62 @@ PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const dou
     rf.R2 = down(R2);
 }
 
-// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).  rf (fine tiles
-// only): the interpolated record.
+// Records of tile j of an object with n points; trk = its FP64 track [n][12] (pos, vel, S).  With rf: the
+// interpolated level-2 record and level 1 from it (r2.U not set); without: r2.U, the one Loewner bound of
+// the tile, and level 1 from that.  cs: table of the frame rotation's cos and sin per point (or null).
 template <int T = TILE>
 PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &r1, TileRec2 &r2,
-                          TileRecF *rf = nullptr) {
+                          TileRecF *rf = nullptr, const double *cs = nullptr) {
     int k0, kp;
-    tile_line<T>(trk, n, j, r1, r2, k0, kp);
-    double qs[T][6];   // rotated covariances of the tile's points
-    // Loewner bounds of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
-    // and, for tiles of more than one point, sum + eta2 I (sum = the first point's + the last point's; over
-    // a tile the covariance grows nearly quadratically in time, so every point's lies close to or below the
-    // chord between the ends, and eta2 is usually 0 even where the ellipsoid shears and eta1 is as large as
-    // its narrow axes).  Level 2 takes the one with the larger tr(U^-1); level 1 the smaller eigenvalue bound.
-    double ref[6], sum[6], first[6];
+    tile_line<T>(trk, n, j, r1, r2, k0, kp, cs);
+    r2.R2 = R2;
+    double qs[T][6], first[6], ref[6], trmax = 0.0;   // rotated covariances of the tile's points
+#pragma unroll
+    for (int i = 0; i < T; i++) {
+        const int k = k0 + i;
+        if (k <= kp) {
+            double c, s;
+            rot_at(cs, k, c, s);
+            rot_cov(trk + (size_t)k * 12 + 6, c, s, qs[i]);
+            trmax = fmax(trmax, qs[i][0] + qs[i][3] + qs[i][5]);
+        }
+    }
     {
-        const double th = OMEGA_PT * kp, t0 = OMEGA_PT * k0;
-        rot_cov(trk + (size_t)kp * 12 + 6, cos(th), sin(th), ref);
+        double c, s;
+        rot_at(cs, kp, c, s);
+        rot_cov(trk + (size_t)kp * 12 + 6, c, s, ref);
+    }
+#pragma unroll
+    for (int u = 0; u < 6; u++) first[u] = qs[0][u];
+    if (rf) {
+        tile_record_f<T>(qs, first, ref, k0, kp, n, r2, R2, trmax, *rf, r1);
+        return;
+    }
+    // One Loewner bound of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
+    // or, for tiles of more than one point, sum + eta2 I (sum = ref + the positive part of the first point's
+    // minus ref), the one with the larger tr(U^-1); level 1 the smaller eigenvalue bound.
+    double sum[6];
+    {
         double d[6], pp[6];
-        rot_cov(trk + (size_t)k0 * 12 + 6, cos(t0), sin(t0), first);
 #pragma unroll
         for (int u = 0; u < 6; u++) d[u] = first[u] - ref[u];
         sym3_pos_part(d, pp);
 #pragma unroll
         for (int u = 0; u < 6; u++) sum[u] = ref[u] + pp[u];
     }
-    double eta1 = 0.0, eta2 = 0.0, trmax = sum[0] + sum[3] + sum[5];
+    double eta1 = 0.0, eta2 = 0.0;
+    trmax = fmax(trmax, sum[0] + sum[3] + sum[5]);
 #pragma unroll
     for (int i = 0; i < T; i++) {
-        const int k = k0 + i;
-        if (k <= kp) {
-            const double th = OMEGA_PT * k;
-            double *q = qs[i];
-            rot_cov(trk + (size_t)k * 12 + 6, cos(th), sin(th), q);
+        if (k0 + i <= kp) {
             double d1[6], d2[6];
 #pragma unroll
             for (int u = 0; u < 6; u++) {
-                d1[u] = q[u] - ref[u];
-                d2[u] = q[u] - sum[u];
+                d1[u] = qs[i][u] - ref[u];
+                d2[u] = qs[i][u] - sum[u];
             }
             eta1 = fmax(eta1, sym3_lmax_bound(d1));
             eta2 = fmax(eta2, sym3_lmax_bound(d2));
-            trmax = fmax(trmax, q[0] + q[3] + q[5]);
         }
     }
     eta1 = eta1 * (1.0 + 1e-12) + 1e-13 * trmax;   // rounding of the rotations, sums and differences
@@ -367,43 +403,6 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     r1.lam = up(fmin(lam_bound(U1), lam_bound(U2)) * (1.0 + 1e-6));
 #pragma unroll
     for (int u = 0; u < 6; u++) r2.U[u] = U[u];
-    r2.R2 = R2;
-    if (rf) tile_record_f<T>(qs, first, ref, k0, kp, n, r2, R2, trmax, *rf);
-}
-
-// Records of the coarse tile J (TILE2 = 2 TILE points) from the records fa, fb of its tiles 2J, 2J + 1 (fb null
-// when the track ends in tile 2J): the axis fitted to its own positions, and U = Ua + (Ub - Ua)_+ + eta I,
-// eta >= the largest eigenvalues of Ua - U and Ub - U, so that U >= Ua, Ub, which bound every point of the two
-// tiles (their kernel term included).
-PHX_HDF void coarse_records(const double *trk, int n, int J, const TileRec2 &fa, const TileRec2 *fb, TileRec1 &r1,
-                            TileRec2 &r2) {
-    int k0, kp;
-    tile_line<TILE2>(trk, n, J, r1, r2, k0, kp);
-    double U[6];
-#pragma unroll
-    for (int u = 0; u < 6; u++) U[u] = fa.U[u];
-    if (fb) {
-        double d[6], pp[6];
-#pragma unroll
-        for (int u = 0; u < 6; u++) d[u] = fb->U[u] - fa.U[u];
-        sym3_pos_part(d, pp);
-#pragma unroll
-        for (int u = 0; u < 6; u++) U[u] += pp[u];
-        double da[6], db[6];
-#pragma unroll
-        for (int u = 0; u < 6; u++) {
-            da[u] = fa.U[u] - U[u];
-            db[u] = fb->U[u] - U[u];
-        }
-        const double tr = fmax(U[0] + U[3] + U[5], fb->U[0] + fb->U[3] + fb->U[5]);
-        const double eta = fmax(0.0, fmax(sym3_lmax_bound(da), sym3_lmax_bound(db))) * (1.0 + 1e-12) + 1e-13 * tr;
-        U[0] += eta; U[3] += eta; U[5] += eta;
-    }
-    r1.lam = up(lam_bound(U) * (1.0 + 1e-6));
-#pragma unroll
-    for (int u = 0; u < 6; u++) r2.U[u] = U[u];
-    r2.R2 = fb ? fmin(fa.R2, fb->R2) : fa.R2;
-    (void)k0; (void)kp;
 }
 
 // The axis parameter range of tile j of a cluster with n points: t in [-4, t1].

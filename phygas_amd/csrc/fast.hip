@@ -7,15 +7,15 @@
 //                    propagated with the FP32 difference force (force32.h) and FP64 accumulators,
 //                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.  Writes the
 //                    AoS tracks and their tiled copy.
-//   3. k_star_recs   per (star, 8-point tile): the tile records of cull.h (axis line, tube radius and
-//                    Loewner covariance bounds in the co-rotating frame: one bound for the tile, and the
-//                    interpolation between its ends for the fine level); k_star_recs_coarse the coarse
-//                    (16-point) records from pairs of them
+//   3. k_star_recs   per (star, 8-point tile) and per (star, 16-point coarse tile): the tile records of
+//                    cull.h (axis line, tube radius and Loewner covariance bounds in the co-rotating frame:
+//                    one bound's largest eigenvalue for level 1, the interpolation between the tile's ends
+//                    for level 2)
 //   4. k_scan_tiles  persistent, length-bucketed: each wavefront takes 64 clusters (sorted by track length)
 //                    x 16 stars, tests every (pair, coarse tile) with the FP32 tube test and the survivors
-//                    with the FP64 Mahalanobis certificate (cull.h), then the tiles of the coarse tiles left
-//                    with the interpolated certificate, and appends the pairs with any tile left, with a bit mask of those
-//                    tiles
+//                    with the Mahalanobis certificate on covariances interpolated over the tile (cull.h), then
+//                    the tiles of the coarse tiles left with the same certificate, and appends the pairs with
+//                    any tile left, with a bit mask of those tiles
 //   5. k_expand      one item per (candidate, tile left), contiguous per candidate
 //      k_items       eight lanes per item: the reference's FP64 evaluation of the tile's points (two
 //                    Mahalanobis forms per point, strict-< first-index minimum)
@@ -59,8 +59,8 @@ struct Cand { int star, cluster; unsigned long long mask; };
 struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 
 // Work buffers (FastWork::mem slots)
-enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_SREC2, W_ITEMS, W_IRES, W_ISTART,
-       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SREC2C, W_SRECF, W_NSLOT };
+enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_ITEMS, W_IRES, W_ISTART,
+       W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SRECFC, W_SRECF, W_NSLOT };
 static_assert(W_NSLOT <= 24, "FastWork holds 24 buffers");
 // Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
 // for the exact evaluation by the finish screen, largest kept segment, level-1 tests of the scan
@@ -70,7 +70,7 @@ enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_L1, C
 // atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
 constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32, C_TOTAL = C_KSEG0 + KSEG * KSEG_STRIDE;
 // Table buffers (FastTables::mem slots)
-enum { T_CTRK, T_CTILED, T_CREC1, T_CREC2, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CREC2C, T_CRECF, T_NSLOT };
+enum { T_CTRK, T_CTILED, T_CREC1, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CRECFC, T_CRECF, T_ROT, T_NSLOT };
 static_assert(T_NSLOT <= 16, "FastTables holds 16 buffers");
 
 static int ensure(void *&p, size_t &have, size_t need) {
@@ -243,10 +243,13 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
 }
 
 // ------------------------------------------------------------------------------------------- tile records
-// One lane per (star, tile): records of the star tracks [ns][npts][12] into srec1 / srec2 / srecf [ns_pad][ntl];
-// rows past the last star hold records no pair test accepts (the scan reads whole blocks of SPW stars).
+// One lane per (star, tile): the level-1 and interpolated level-2 records of the star tracks [ns][npts][12],
+// srec1 / srecf [ns_pad][ntl] of the 8-point tiles (Tl = TILE) and the same of the 16-point coarse tiles
+// (Tl = TILE2, [ns_pad][ntlc]); rows past the last star hold records no pair test accepts (the scan reads
+// whole blocks of SPW stars).
+template <int Tl>
 __global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ trk, int npts, int ntl, int n, int n_pad,
-                                                   TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2,
+                                                   const double *__restrict__ cs, TileRec1 *__restrict__ r1,
                                                    TileRecF *__restrict__ rf) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)n_pad * ntl) return;
@@ -254,28 +257,9 @@ __global__ __launch_bounds__(256) void k_star_recs(const double *__restrict__ tr
     TileRec1 a = pad_rec1();
     TileRec2 b = {};
     TileRecF f = {};
-    if (s < n) tile_records(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b, &f);
+    if (s < n) tile_records<Tl>(trk + (size_t)s * npts * 12, npts, j, 0.0, a, b, &f, cs);
     r1[id] = a;
-    r2[id] = b;
     rf[id] = f;
-}
-
-// Coarse records (TILE2 points: tiles 2J, 2J + 1) of the stars from their tile records r2 [n_pad][ntl]:
-// rc1 / rc2 [n_pad][ntlc]; padding rows as k_star_recs's.
-__global__ __launch_bounds__(256) void k_star_recs_coarse(const double *__restrict__ trk, int npts, int ntl, int ntlc,
-                                                          int n, int n_pad, const TileRec2 *__restrict__ r2,
-                                                          TileRec1 *__restrict__ rc1, TileRec2 *__restrict__ rc2) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)n_pad * ntlc) return;
-    const int s = (int)(id / ntlc), J = (int)(id % ntlc);
-    TileRec1 a = pad_rec1();
-    TileRec2 b = {};
-    if (s < n) {
-        const TileRec2 *f = r2 + (size_t)s * ntl + 2 * J;
-        coarse_records(trk + (size_t)s * npts * 12, npts, J, f[0], 2 * J + 1 < ntl ? f + 1 : nullptr, a, b);
-    }
-    rc1[id] = a;
-    rc2[id] = b;
 }
 
 // Cluster tables for the fast form: AoS FP64 tracks [c][npts][12].
@@ -292,13 +276,13 @@ __global__ __launch_bounds__(256) void k_cluster_aos(const double *__restrict__ 
     for (int q = 0; q < 12; q++) o[q] = v[q];
 }
 
-// Cluster tile records in structure-of-arrays [tile][nc_pad] over clusters sorted by track length (longest
-// first); the pair's kernel radius enters through the cluster (R^2 I added to U).
+// Cluster tile records (Tl = TILE or TILE2 points) in structure-of-arrays [tile][nc_pad] over clusters sorted
+// by track length (longest first); the pair's kernel radius enters through the cluster (R^2 I added).
+template <int Tl>
 __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__ ctrk, const int *__restrict__ cn,
                                                       const double *__restrict__ R2, const int *__restrict__ perm,
-                                                      int nc, int nc_pad, int npts, int ntl,
-                                                      TileRec1 *__restrict__ r1, TileRec2 *__restrict__ r2,
-                                                      TileRecF *__restrict__ rf) {
+                                                      int nc, int nc_pad, int npts, int ntl, const double *__restrict__ rot,
+                                                      TileRec1 *__restrict__ r1, TileRecF *__restrict__ rf) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)nc_pad * ntl) return;
     const int j = (int)(id / nc_pad), cs = (int)(id % nc_pad);
@@ -307,32 +291,10 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
     TileRecF f = {};
     if (cs < nc) {
         const int c = perm[cs], n = cn[c];
-        if (j * TILE < n) tile_records(ctrk + (size_t)c * npts * 12, n, j, R2[c], a, b, &f);
+        if (j * Tl < n) tile_records<Tl>(ctrk + (size_t)c * npts * 12, n, j, R2[c], a, b, &f, rot);
     }
     r1[id] = a;
-    r2[id] = b;
     rf[id] = f;
-}
-
-// Coarse records of the clusters from their tile records r2 [ntl][nc_pad] (sorted order): rc1 / rc2
-// [ntlc][nc_pad].
-__global__ __launch_bounds__(256) void k_cluster_recs_coarse(const double *__restrict__ ctrk, const int *__restrict__ cn,
-                                                             const int *__restrict__ perm, int nc, int nc_pad, int npts,
-                                                             int ntlc, const TileRec2 *__restrict__ r2,
-                                                             TileRec1 *__restrict__ rc1, TileRec2 *__restrict__ rc2) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long)nc_pad * ntlc) return;
-    const int J = (int)(id / nc_pad), cs = (int)(id % nc_pad);
-    TileRec1 a = {};
-    TileRec2 b = {};
-    if (cs < nc) {
-        const int c = perm[cs], n = cn[c];
-        if (2 * J * TILE < n)
-            coarse_records(ctrk + (size_t)c * npts * 12, n, J, r2[(size_t)(2 * J) * nc_pad + cs],
-                           (2 * J + 1) * TILE < n ? r2 + (size_t)(2 * J + 1) * nc_pad + cs : nullptr, a, b);
-    }
-    rc1[id] = a;
-    rc2[id] = b;
 }
 
 // ------------------------------------------------------------------------------------------- tile scan
@@ -348,39 +310,32 @@ __global__ __launch_bounds__(256) void k_cluster_recs_coarse(const double *__res
 //            cluster's lane as (star, tile) bits.
 // A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
 // drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
-// Level-2 records of a wavefront's 64 clusters in LDS, field-major ([field][lane], the 14 fields l2_pass
-// reads): the lanes' writes are consecutive and reads of different clusters fall in different banks, where a
-// [lane] array of 128-byte records puts every lane's copy of a field in one bank.
-constexpr int C2F = 14;
-__device__ __forceinline__ void lds_put_rec2(double (*d)[64], int i, const TileRec2 &r) {
+// Coarse level-2 records of a wavefront's 64 clusters in LDS, field-major ([field][lane], the 24 FP32 fields
+// of TileRecF): the lanes' writes are consecutive and reads of different clusters fall in different banks,
+// where a [lane] array of 96-byte records would put the lanes' copies of a field in few banks.
+constexpr int CFF = sizeof(TileRecF) / sizeof(float);
+static_assert(CFF == 24, "TileRecF is 24 floats");
+__device__ __forceinline__ void lds_put_recf(float (*d)[64], int i, const TileRecF &r) {
+    const float *v = reinterpret_cast<const float *>(&r);
 #pragma unroll
-    for (int u = 0; u < 3; u++) { d[u][i] = r.a[u]; d[3 + u][i] = r.b[u]; }
-#pragma unroll
-    for (int u = 0; u < 6; u++) d[6 + u][i] = r.U[u];
-    d[12][i] = r.e;
-    d[13][i] = r.R2;
+    for (int u = 0; u < CFF; u++) d[u][i] = v[u];
 }
-__device__ __forceinline__ TileRec2 lds_get_rec2(const double (*d)[64], int i) {
-    TileRec2 r;
+__device__ __forceinline__ TileRecF lds_get_recf(const float (*d)[64], int i) {
+    TileRecF r;
+    float *v = reinterpret_cast<float *>(&r);
 #pragma unroll
-    for (int u = 0; u < 3; u++) { r.a[u] = d[u][i]; r.b[u] = d[3 + u][i]; }
-#pragma unroll
-    for (int u = 0; u < 6; u++) r.U[u] = d[6 + u][i];
-    r.e = d[12][i];
-    r.R2 = d[13][i];
-    r.t1 = 0.0;
-    r.pad = 0.0;
+    for (int u = 0; u < CFF; u++) v[u] = d[u][i];
     return r;
 }
 
 template <typename MaskT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
-                                                    const TileRec2 *__restrict__ crec2c,
+                                                    const TileRecF *__restrict__ crecfc,
                                                     const TileRecF *__restrict__ crecf,
                                                     const int *__restrict__ ntc_sorted,
                                                     const int *__restrict__ cn_sorted, const int *__restrict__ perm,
                                                     int nc, int nc_pad, const TileRec1 *__restrict__ srec1c,
-                                                    const TileRec2 *__restrict__ srec2c,
+                                                    const TileRecF *__restrict__ srecfc,
                                                     const TileRecF *__restrict__ srecf, int ns, int ntl, int ntlc,
                                                     float kchi, double chi2c, Cand *__restrict__ cand,
                                                     unsigned long long *__restrict__ ncand, long cap,
@@ -388,14 +343,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
     __shared__ unsigned short q_s[4][SPW * 64];
-    __shared__ double c2_s[4][C2F][64];
-    __shared__ float t1_s[4][64];
+    __shared__ float c2_s[4][CFF][64];
     __shared__ unsigned l2_s[4][64];
-    __shared__ int nt_s[4][64], ncl_s[4][64];
+    __shared__ int nt_s[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const float chi2cf = up(chi2c);   // the fine level's FP32 cut, rounded up
     unsigned short *q = q_s[w];
-    double (*c2s)[64] = c2_s[w];
+    float (*c2s)[64] = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
     const int ntasks = ncb * nsb;
     // exclusive prefix of a per-lane count over the wavefront; the total into *tot
@@ -430,7 +384,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int s0 = sb * SPW;
         const int nsv = min(SPW, ns - s0);   // stars of the block (the rest is padding)
         nt_s[w][lane] = nt;
-        ncl_s[w][lane] = ncl;
         unsigned n_l1 = 0, n_l2 = 0;   // tests of this task (wave-uniform)
         unsigned fine_l2 = 0;          // fine level-2 tests of this lane's queue entries
         MaskT mask[SPW];
@@ -458,16 +411,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (nq == 0) continue;
             n_l2 += nq;
             enqueue(l1, o1);
-            if (l1) lds_put_rec2(c2s, lane, crec2c[(size_t)J * nc_pad + cs]);
-            t1_s[w][lane] = t1;
+            if (l1) lds_put_recf(c2s, lane, crecfc[(size_t)J * nc_pad + cs]);
             l2_s[w][lane] = 0u;
             wave_sync();
             for (int e0 = 0; e0 < nq; e0 += 64) {   // wave-uniform
                 const int e = e0 + lane;
                 if (e < nq) {
                     const int ent = q[e], i = ent >> 6, c = ent & 63;
-                    const TileRec2 s2 = srec2c[(size_t)(s0 + i) * ntlc + J];
-                    if (l2_pass<TILE2>(s2, lds_get_rec2(c2s, c), (double)t1_s[w][c], chi2c)) atomicOr(&l2_s[w][c], 1u << i);
+                    if (l2f_pass<TILE2>(srecfc[(size_t)(s0 + i) * ntlc + J], lds_get_recf(c2s, c), chi2cf))
+                        atomicOr(&l2_s[w][c], 1u << i);
                 }
             }
             wave_sync();
@@ -484,7 +436,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const int e = e0 + lane;
                 if (e < nq2) {
                     const int ent = q[e], i = ent >> 6, c = ent & 63;
-                    const int ntc = nt_s[w][c], nclc = ncl_s[w][c], csc = cb * 64 + c;
+                    const int ntc = nt_s[w][c], csc = cb * 64 + c;
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
                         const int j = 2 * J + h;
@@ -1238,8 +1190,16 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
     T.ntiles = n_tiles(npts);
     const long ntiled = (long)nc * T.ntiles * TREC;
     if (ensure(T.mem[T_CTRK], T.bytes[T_CTRK], sizeof(double) * 12 * (size_t)nc * npts) ||
-        ensure(T.mem[T_CTILED], T.bytes[T_CTILED], sizeof(double) * (size_t)ntiled))
+        ensure(T.mem[T_CTILED], T.bytes[T_CTILED], sizeof(double) * (size_t)ntiled) ||
+        ensure(T.mem[T_ROT], T.bytes[T_ROT], sizeof(double) * 2 * (size_t)npts))
         return 1;
+    // the tile records' frame rotation per point (the stream is synchronised before the vector goes)
+    std::vector<double> rot(2 * (size_t)npts);
+    for (int k = 0; k < npts; k++) {
+        rot[2 * k] = cos(OMEGA_PT * k);
+        rot[2 * k + 1] = sin(OMEGA_PT * k);
+    }
+    PHX_CK(hipMemcpyAsync(T.mem[T_ROT], rot.data(), sizeof(double) * 2 * (size_t)npts, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_cluster_aos, dim3(blocks_for((long)nc * npts, 256)), dim3(256), 0, st, d_csoa, d_cn, nc, npts,
                        (double *)T.mem[T_CTRK]);
     hipLaunchKernelGGL(k_tiled, dim3(blocks_for(ntiled, 256)), dim3(256), 0, st, (const double *)T.mem[T_CTRK], d_cn,
@@ -1261,9 +1221,8 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
         ensure(T.mem[T_CPERM], T.bytes[T_CPERM], 4 * (size_t)T.nc_pad) ||
         ensure(T.mem[T_CNS], T.bytes[T_CNS], 4 * (size_t)T.nc_pad) ||
         ensure(T.mem[T_CREC1], T.bytes[T_CREC1], sizeof(TileRec1) * nrec) ||
-        ensure(T.mem[T_CREC2], T.bytes[T_CREC2], sizeof(TileRec2) * nrec) ||
         ensure(T.mem[T_CREC1C], T.bytes[T_CREC1C], sizeof(TileRec1) * nrec_c) ||
-        ensure(T.mem[T_CREC2C], T.bytes[T_CREC2C], sizeof(TileRec2) * nrec_c) ||
+        ensure(T.mem[T_CRECFC], T.bytes[T_CRECFC], sizeof(TileRecF) * nrec_c) ||
         ensure(T.mem[T_CRECF], T.bytes[T_CRECF], sizeof(TileRecF) * nrec))
         return 1;
     nt_sorted.resize(T.nc_pad, 0);
@@ -1272,13 +1231,13 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
     PHX_CK(hipMemcpyAsync(T.mem[T_CNT], nt_sorted.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
     PHX_CK(hipMemcpyAsync(T.mem[T_CNS], cn_sorted.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
     PHX_CK(hipMemcpyAsync(T.mem[T_CPERM], perm.data(), 4 * (size_t)T.nc_pad, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_cluster_recs, dim3(blocks_for((long)nrec, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_cluster_recs<TILE>, dim3(blocks_for((long)nrec, 256)), dim3(256), 0, st,
                        (const double *)T.mem[T_CTRK], d_cn, d_R2, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
-                       T.ntiles, (TileRec1 *)T.mem[T_CREC1], (TileRec2 *)T.mem[T_CREC2], (TileRecF *)T.mem[T_CRECF]);
-    hipLaunchKernelGGL(k_cluster_recs_coarse, dim3(blocks_for((long)nrec_c, 256)), dim3(256), 0, st,
-                       (const double *)T.mem[T_CTRK], d_cn, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
-                       n_coarse(T.ntiles), (const TileRec2 *)T.mem[T_CREC2], (TileRec1 *)T.mem[T_CREC1C],
-                       (TileRec2 *)T.mem[T_CREC2C]);
+                       T.ntiles, (const double *)T.mem[T_ROT], (TileRec1 *)T.mem[T_CREC1], (TileRecF *)T.mem[T_CRECF]);
+    hipLaunchKernelGGL(k_cluster_recs<TILE2>, dim3(blocks_for((long)nrec_c, 256)), dim3(256), 0, st,
+                       (const double *)T.mem[T_CTRK], d_cn, d_R2, (const int *)T.mem[T_CPERM], nc, T.nc_pad, npts,
+                       n_coarse(T.ntiles), (const double *)T.mem[T_ROT], (TileRec1 *)T.mem[T_CREC1C],
+                       (TileRecF *)T.mem[T_CRECFC]);
     PHX_CK(hipGetLastError());
     PHX_CK(hipStreamSynchronize(st));
     T.d_cn = d_cn;
@@ -1299,9 +1258,8 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
         ensure(W.mem[W_TRK], W.bytes[W_TRK], sizeof(double) * 12 * (size_t)ns * npts) ||
         ensure(W.mem[W_TILED], W.bytes[W_TILED], sizeof(double) * TREC * (size_t)ns * ntl) ||
         ensure(W.mem[W_SREC1], W.bytes[W_SREC1], sizeof(TileRec1) * (size_t)ns_pad(ns) * ntl) ||
-        ensure(W.mem[W_SREC2], W.bytes[W_SREC2], sizeof(TileRec2) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_SREC1C], W.bytes[W_SREC1C], sizeof(TileRec1) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
-        ensure(W.mem[W_SREC2C], W.bytes[W_SREC2C], sizeof(TileRec2) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
+        ensure(W.mem[W_SRECFC], W.bytes[W_SRECFC], sizeof(TileRecF) * (size_t)ns_pad(ns) * n_coarse(ntl)) ||
         ensure(W.mem[W_SRECF], W.bytes[W_SRECF], sizeof(TileRecF) * (size_t)ns_pad(ns) * ntl) ||
         ensure(W.mem[W_CAND], W.bytes[W_CAND], sizeof(Cand) * (size_t)W.cand_cap) ||
         ensure(W.mem[W_KEPT], W.bytes[W_KEPT], sizeof(Kept) * (size_t)W.kept_cap) ||
@@ -1328,7 +1286,6 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
     TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
-    TileRec2 *srec2 = (TileRec2 *)W.mem[W_SREC2];
     if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) return 1;
     if (mark) mark(mark_ctx, "draws");
     if (a.exact_tracks) {   // the plain FP64 tracks: rows identical to the plain form's
@@ -1343,20 +1300,22 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     }
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
-    hipLaunchKernelGGL(k_star_recs, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts, ntl, ns,
-                       ns_pad(ns), srec1, srec2, (TileRecF *)W.mem[W_SRECF]);
-    hipLaunchKernelGGL(k_star_recs_coarse, dim3(blocks_for((long)ns_pad(ns) * n_coarse(ntl), 256)), dim3(256), 0, st,
-                       trk, npts, ntl, n_coarse(ntl), ns, ns_pad(ns), (const TileRec2 *)srec2,
-                       (TileRec1 *)W.mem[W_SREC1C], (TileRec2 *)W.mem[W_SREC2C]);
+    PHX_REQUIRE(npts <= T.npts, "star tracks longer than the cluster tables");
+    const double *rot = (const double *)T.mem[T_ROT];
+    hipLaunchKernelGGL(k_star_recs<TILE>, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts,
+                       ntl, ns, ns_pad(ns), rot, srec1, (TileRecF *)W.mem[W_SRECF]);
+    hipLaunchKernelGGL(k_star_recs<TILE2>, dim3(blocks_for((long)ns_pad(ns) * n_coarse(ntl), 256)), dim3(256), 0, st,
+                       trk, npts, n_coarse(ntl), ns, ns_pad(ns), rot, (TileRec1 *)W.mem[W_SREC1C],
+                       (TileRecF *)W.mem[W_SRECFC]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
     auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
     hipLaunchKernelGGL(scan, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
-                       (const TileRec1 *)T.mem[T_CREC1C], (const TileRec2 *)T.mem[T_CREC2C],
+                       (const TileRec1 *)T.mem[T_CREC1C], (const TileRecF *)T.mem[T_CRECFC],
                        (const TileRecF *)T.mem[T_CRECF], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
-                       (const TileRec2 *)W.mem[W_SREC2C], (const TileRecF *)W.mem[W_SRECF], ns, ntl, n_coarse(ntl),
+                       (const TileRecF *)W.mem[W_SRECFC], (const TileRecF *)W.mem[W_SRECF], ns, ntl, n_coarse(ntl),
                        up(sqrt(chi2c)), chi2c,
                        (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK, cnt + C_L1, cnt + C_L2);
     PHX_CK(hipGetLastError());

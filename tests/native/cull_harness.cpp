@@ -14,19 +14,25 @@ extern "C" int cc_scan(const double *strk, int ns, int snpts, const double *ctrk
     TileRec2 *s2 = new TileRec2[(size_t)ns * ntl];
     TileRecF *sf = new TileRecF[(size_t)ns * ntl];
     for (int s = 0; s < ns; s++)
-        for (int j = 0; j < ntl; j++)
+        for (int j = 0; j < ntl; j++) {
+            TileRec1 u1;
+            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, u1, s2[(size_t)s * ntl + j]);   // one U
             tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j],
                          &sf[(size_t)s * ntl + j]);
+        }
     *n_l2u = 0;
     const double chi2c = chi2_max * (1.0 + CUT_REL);
     const float kchi = up(sqrt(chi2c));
     for (int c = 0; c < nc; c++) {
         const int n = cn[c], nt = (n + TILE - 1) / TILE;
         for (int j = 0; j < ntl; j++) {
-            TileRec1 c1 = {};
+            TileRec1 c1 = {}, u1;
             TileRec2 c2 = {};
             TileRecF cf = {};
-            if (j < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], c1, c2, &cf);
+            if (j < nt) {
+                tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], u1, c2);   // one U
+                tile_records(ctrk + (size_t)c * cnpts * 12, n, j, R2[c], c1, c2, &cf);
+            }
             const float t1 = tile_t1(n, j);
             for (int s = 0; s < ns; s++) {
                 const size_t o = ((size_t)s * nc + c) * ntl + j;
@@ -65,56 +71,53 @@ extern "C" long cc_pad_passes(const double *ctrk, const int *cn, const double *R
     return passes;
 }
 
-// The scan's two-level order: a coarse tile (TILE2 points) through level 1 and level 2 first, then each of
-// its two tiles through both levels when the coarse tile is left.  p2 as cc_scan's (tiles left);
-// counts[0..3] = level-1 tests, level-2 tests, coarse level-1 tests, coarse level-2 tests.
+// The scan's two-level order: a coarse tile (TILE2 points) through level 1 and the interpolated level 2
+// first, then each of its two tiles through the fine level 2 when the coarse tile is left.  p2 as cc_scan's
+// (tiles left); counts[0..3] = coarse level-1 tests, coarse level-2 tests, fine level-2 tests, fine tiles left.
 extern "C" int cc_scan2(const double *strk, int ns, int snpts, const double *ctrk, const int *cn, const double *R2,
                         int nc, int cnpts, double chi2_max, unsigned char *p2, long *counts) {
     const int ntl = (snpts + TILE - 1) / TILE, ntl2 = (snpts + TILE2 - 1) / TILE2;
     TileRec1 *s1 = new TileRec1[(size_t)ns * ntl], *q1 = new TileRec1[(size_t)ns * ntl2];
-    TileRec2 *s2 = new TileRec2[(size_t)ns * ntl], *q2 = new TileRec2[(size_t)ns * ntl2];
-    TileRecF *sf = new TileRecF[(size_t)ns * ntl];
+    TileRecF *sf = new TileRecF[(size_t)ns * ntl], *qf = new TileRecF[(size_t)ns * ntl2];
     for (int s = 0; s < ns; s++) {
-        for (int j = 0; j < ntl; j++)
-            tile_records(strk + (size_t)s * snpts * 12, snpts, j, 0.0, s1[(size_t)s * ntl + j], s2[(size_t)s * ntl + j],
-                         &sf[(size_t)s * ntl + j]);
-        for (int J = 0; J < ntl2; J++) {
-            const TileRec2 *f = s2 + (size_t)s * ntl + 2 * J;
-            coarse_records(strk + (size_t)s * snpts * 12, snpts, J, f[0], 2 * J + 1 < ntl ? f + 1 : nullptr,
-                           q1[(size_t)s * ntl2 + J], q2[(size_t)s * ntl2 + J]);
-        }
+        const double *t = strk + (size_t)s * snpts * 12;
+        TileRec2 r2;
+        for (int j = 0; j < ntl; j++) tile_records(t, snpts, j, 0.0, s1[(size_t)s * ntl + j], r2, &sf[(size_t)s * ntl + j]);
+        for (int J = 0; J < ntl2; J++)
+            tile_records<TILE2>(t, snpts, J, 0.0, q1[(size_t)s * ntl2 + J], r2, &qf[(size_t)s * ntl2 + J]);
     }
     const double chi2c = chi2_max * (1.0 + CUT_REL);
-    const float kchi = up(sqrt(chi2c));
+    const float kchi = up(sqrt(chi2c)), chi2cf = up(chi2c);
     for (int i = 0; i < 4; i++) counts[i] = 0;
     for (int c = 0; c < nc; c++) {
         const int n = cn[c], nt = (n + TILE - 1) / TILE, nt2 = (n + TILE2 - 1) / TILE2;
+        const double *t = ctrk + (size_t)c * cnpts * 12;
         for (int J = 0; J < ntl2; J++) {
-            TileRec1 c1 = {}, d1[2] = {};
-            TileRec2 c2 = {}, d2[2] = {};
-            TileRecF df[2] = {};
+            TileRec1 c1 = {}, d1;
+            TileRec2 r2;
+            TileRecF cq = {}, df[2] = {};
             for (int h = 0; h < 2; h++)
-                if (2 * J + h < nt) tile_records(ctrk + (size_t)c * cnpts * 12, n, 2 * J + h, R2[c], d1[h], d2[h], &df[h]);
-            if (J < nt2) coarse_records(ctrk + (size_t)c * cnpts * 12, n, J, d2[0], 2 * J + 1 < nt ? &d2[1] : nullptr, c1, c2);
+                if (2 * J + h < nt) tile_records(t, n, 2 * J + h, R2[c], d1, r2, &df[h]);
+            if (J < nt2) tile_records<TILE2>(t, n, J, R2[c], c1, r2, &cq);
             for (int s = 0; s < ns; s++) {
                 for (int h = 0; h < 2; h++)
                     if (2 * J + h < ntl) p2[((size_t)s * nc + c) * ntl + 2 * J + h] = 0;
                 if (J >= nt2) continue;
-                counts[2]++;
+                counts[0]++;
                 if (!l1_pass<TILE2>(q1[(size_t)s * ntl2 + J], c1, tile_t1<TILE2>(n, J), kchi)) continue;
-                counts[3]++;
-                if (!l2_pass<TILE2>(q2[(size_t)s * ntl2 + J], c2, (double)tile_t1<TILE2>(n, J), chi2c)) continue;
+                counts[1]++;
+                if (!l2f_pass<TILE2>(qf[(size_t)s * ntl2 + J], cq, chi2cf)) continue;
                 for (int h = 0; h < 2; h++) {
                     const int j = 2 * J + h;
                     if (j >= nt) continue;
-                    counts[0]++;
-                    if (!l1_pass(s1[(size_t)s * ntl + j], d1[h], tile_t1(n, j), kchi)) continue;
-                    counts[1]++;
-                    p2[((size_t)s * nc + c) * ntl + j] = l2f_pass(sf[(size_t)s * ntl + j], df[h], up(chi2c));
+                    counts[2]++;
+                    const bool f = l2f_pass(sf[(size_t)s * ntl + j], df[h], chi2cf);
+                    p2[((size_t)s * nc + c) * ntl + j] = f;
+                    counts[3] += f;
                 }
             }
         }
     }
-    delete[] s1; delete[] s2; delete[] q1; delete[] q2; delete[] sf;
+    delete[] s1; delete[] q1; delete[] sf; delete[] qf;
     return ntl;
 }

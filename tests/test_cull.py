@@ -121,7 +121,9 @@ def test_dropped_tiles_hold_no_point_under_the_cut(harness, oracle, workload):
     assert qt[p2c == 0].min() >= CHI2 * (1 - 1e-9)
     assert np.all(p2c <= p2)                       # only ever drops more
     assert np.all(p2c[ks, kc, kb[ks, kc] // 8] == 1)
-    assert counts[2] < 0.6 * np.broadcast_to(exists, p1.shape).sum()   # coarse level-1 tests: about half
+    assert counts[0] < 0.6 * np.broadcast_to(exists, p1.shape).sum()   # coarse level-1 tests: about half
+    assert counts[2] < 0.45 * counts[1]            # the coarse level 2 leaves a minority of the fine tests
+    assert counts[3] == p2c.sum()
     # the record of a star slot past a batch's end passes level 1 against no cluster tile
     harness.cc_pad_passes.restype = C.c_long
     harness.cc_pad_passes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]

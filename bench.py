@@ -36,7 +36,7 @@ METRIC = "star-cluster pairs/s, chosen vs plain GPU at |dp_r50|<=1e-4; p50/p99 l
 PLAIN_POINT_FLOP = 126.0   # k_chi2_plain, one interior pair-point (56 at a track's last point)
 PLAIN_LAST_FLOP = 56.0
 SCAN_L1_FLOP = 76.0        # cull.h l1_pass, one (pair, tile) test, FP32
-SCAN_L2_FLOP = 179.0       # cull.h l2_pass, one (pair, tile) certificate, FP64
+SCAN_L2_FLOP = 367.0       # cull.h l2f_pass, one (pair, tile) certificate, FP32 (both quadratic checks, vertex)
 POINT_FLOP = 134.0         # k_items, one pair-point: both Mahalanobis forms and the segment parameter, FP64
 
 

@@ -117,6 +117,7 @@ __device__ __forceinline__ double rcp64(double b) {
 // from one log2 and two exp2; every operation in FP64 (a few ulp from the reference's pow-based evaluation).
 // inv_ha = 1 / halo_a.
 __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
+#pragma clang fp contract(fast)
     const double R2 = p.x * p.x + p.y * p.y, r2 = R2 + p.z * p.z;
     const double iu = rsq64(r2 + P.bulge_b * P.bulge_b);
     const double fb = -P.bulge_gm * (iu * iu * iu);
@@ -141,6 +142,7 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
 __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
                                                   double *__restrict__ trk, Pot P, PotF Q, int npts, int n,
                                                   double *__restrict__ tiled, int ntl) {
+#pragma clang fp contract(fast)
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = gid >> 1, q = gid & 1;
     const bool live = s < n;

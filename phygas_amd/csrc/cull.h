@@ -417,8 +417,11 @@ PHX_HDF float f_sqrt_fast(float x) { return f_sqrt_hw(x); }
 
 // Level 1, FP32.  kchi = sqrt(chi2_max (1 + CUT_REL)) rounded up.  The test accepts when
 // dist(0, axis segment) <= rho + kchi sqrt(lam_s + lam_c), compared as squares with relative margin 4e-5
-// and an absolute one of 1e-5 kpc (plus 2e-6 |da|^2 for the cancellation in the cross product); FP32
-// rounding of every step is far below both.
+// and an absolute one of 1e-5 kpc (plus 2e-6 |da|^2 for the rounding of the differences); FP32 rounding of
+// every step is far below both.  The distance is |da + db t~|^2 at t~, the closest parameter -da.db / |db|^2
+// through the hardware reciprocal, clamped to the segment: |t~ - t*| <= delta = 1e-6 (|t~| + 1), and the
+// excess of the squared distance at t~ over the minimum is at most 3 |db|^2 delta^2 (the derivative at a
+// clamped end is at most 2 |db|^2 delta when t* lies within delta of it), which is taken off.
 template <int T = TILE>
 PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi) {
     const float dax = s.ax - c.ax, day = s.ay - c.ay, daz = s.az - c.az;
@@ -426,21 +429,16 @@ PHX_HDF bool l1_pass(const TileRec1 &s, const TileRec1 &c, float t1, float kchi)
     const float t0 = -0.5f * T;
     const float dd = f_fma(dbx, dbx, f_fma(dby, dby, dbz * dbz));
     const float ad = f_fma(dax, dbx, f_fma(day, dby, daz * dbz));
-    const float x0x = f_fma(dbx, t0, dax), x0y = f_fma(dby, t0, day), x0z = f_fma(dbz, t0, daz);
-    const float x1x = f_fma(dbx, t1, dax), x1y = f_fma(dby, t1, day), x1z = f_fma(dbz, t1, daz);
-    const float d0 = f_fma(x0x, x0x, f_fma(x0y, x0y, x0z * x0z));
-    const float d1 = f_fma(x1x, x1x, f_fma(x1y, x1y, x1z * x1z));
-    const float cx = f_fma(day, dbz, -daz * dby), cy = f_fma(daz, dbx, -dax * dbz), cz = f_fma(dax, dby, -day * dbx);
-    const float dl = f_fma(cx, cx, f_fma(cy, cy, cz * cz));   // |da x db|^2 = dd * (distance of the line)^2
-    const bool interior = (dd > 0.0f) & (-ad > t0 * dd) & (-ad < t1 * dd);
+    const float tt = fminf(fmaxf(-ad * f_rcp(dd), t0), t1);   // dd = 0: NaN or inf, clamped to an end
+    const float xx = f_fma(dbx, tt, dax), xy = f_fma(dby, tt, day), xz = f_fma(dbz, tt, daz);
+    const float dl = fabsf(tt) * 1e-6f + 1e-6f;
+    const float d2 = f_fma(xx, xx, f_fma(xy, xy, xz * xz)) - 3.0f * dd * dl * dl;
     const float e = s.e + c.e;
     const float na = fabsf(dax) + fabsf(day) + fabsf(daz), nb = fabsf(dbx) + fabsf(dby) + fabsf(dbz);
     const float rho = e + (float)ROT_W * (na + 0.5f * T * nb + e);
     const float lim = (rho + kchi * f_sqrt_fast(s.lam + c.lam)) * 1.00001f + 1e-5f;
     const float lim2 = f_fma(lim, lim, 2e-6f * na * na);
-    const float lhs = interior ? dl : fminf(d0, d1);
-    const float rhs = interior ? lim2 * dd : lim2;
-    return lhs * 0.99996f <= rhs;
+    return d2 * 0.99996f <= lim2;
 }
 
 // Level 2, FP64.  chi2c = chi2_max (1 + CUT_REL).  For y = s z (any z, s > 0) the dual bound is

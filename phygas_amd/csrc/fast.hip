@@ -330,6 +330,7 @@ __device__ __forceinline__ TileRecF lds_get_recf(const float (*d)[64], int i) {
     return r;
 }
 
+constexpr int L1G = 2;   // star records of the level-1 loop held at once (scalar registers)
 template <typename MaskT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
                                                     const TileRecF *__restrict__ crecfc,
@@ -398,12 +399,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             n_l1 += (unsigned)nsv * (unsigned)__popcll(__ballot(cv));
             unsigned l1 = 0u;
 #pragma unroll
-            for (int i0 = 0; i0 < SPW; i0 += 8) {   // star records of the padded block: wave-uniform loads
-                TileRec1 sr[8];
+            for (int i0 = 0; i0 < SPW; i0 += L1G) {   // star records of the padded block: wave-uniform loads
+                TileRec1 sr[L1G];
 #pragma unroll
-                for (int u = 0; u < 8; u++) sr[u] = srec1c[(size_t)(s0 + i0 + u) * ntlc + J];
+                for (int u = 0; u < L1G; u++) sr[u] = srec1c[(size_t)(s0 + i0 + u) * ntlc + J];
 #pragma unroll
-                for (int u = 0; u < 8; u++)
+                for (int u = 0; u < L1G; u++)
                     if (l1_pass<TILE2>(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
             }
             if (!cv) l1 = 0u;

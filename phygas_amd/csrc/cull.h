@@ -1,4 +1,4 @@
-// Tile culling of the fast screening scan: per-object tile records and the two pair tests.
+// Tile culling of the fast screening scan: per-object tile records and the pair tests.
 //
 // A tile j of an object holds the points k in [8j, 8j+8) of its track (the points whose chi2 the
 // reference evaluates: reference src/phygas.c:395-403 / src/phygas_kernel_body.h:124-163), and the
@@ -7,36 +7,34 @@
 // Galactic z axis, so that the circular part of the motion drops out and a tile's positions lie close to
 // a straight line.
 //
-// Record of an object tile (built from its FP64 track):
-//   a, b   least-squares line p'_k ~ a + b (k - tc), tc = 8j + 4, through the rotated positions
+// Records of an object tile (built from its FP64 track; tiles of 8 points, and coarse tiles of 16 with the
+// same construction):
+//   a, b   least-squares line p'_k ~ a + b (k - tc), tc = k0 + T/2, through the rotated positions
 //   e      largest distance of a rotated position from that line (the tube radius)
-//   U      3x3 matrix with U >= R_k S_k R_k^T for every point of the tile (Loewner order): either the
-//          rotated covariance of the tile's last point plus eta1 I, or the sum of the first and the last
-//          point's plus eta2 I, eta >= the largest eigenvalue of R_k S_k R_k^T minus that base over the tile
-//          (the one with the larger tr(U^-1)); clusters add the pair's kernel R^2 I
-//   lam    upper bound of the largest eigenvalue of U (the smaller of trace and Gershgorin bound, of the
-//          two choices)
+//   F, L   the rotated covariances of the tile's first and last points, each plus a slack E (the positive
+//          part of the interpolation's largest error, along the ellipsoid's long axes), and eta, so that
+//          R_k S_k R_k^T <= (1 - w_k) F + w_k L + eta I for every point (Loewner order), w_k = (k - k0) / np;
+//          clusters add the pair's kernel R^2 I (TileRecF, stored in FP32 with its rounding in e and eta)
+//   lam    upper bound of the largest eigenvalue over the tile: max(lam(F), lam(L)) + eta (TileRec1)
+// (tile_records without TileRecF builds the older single bound U >= every R_k S_k R_k^T instead: the last
+// point's covariance plus eta1 I, or last + (first - last)_+ plus eta2 I; the tests compare against it.)
 //
 // For a pair (star s, cluster c) and a point k of tile j, the reference's value is
 // chi2_k = min(q(m0), q(ms)) with q(x) = x^T M_k^-1 x, M_k = S_s,k + S_c,k + R^2 I, m0 = p_s,k - p_c,k and
 // ms a point of the segment m0 -> m1 = p_s,k+1 - p_c,k+1.  Rotated by R_k, m0 lies in the tube
-// T = {da + db t + r : t in [-4, t1], |r| <= rho}, da = a_s - a_c, db = b_s - b_c, rho = e_s + e_c; m1 rotated
-// by R_k+1 lies in T too, so rotated by R_k it lies in R_1^-1 T (R_1 the rotation by OMEGA_PT), within
-// ROT_W |m1| of T.  Every point of the segment is a convex combination of its ends, and
-// q(x) >= x^T U^-1 x with U = U_s + U_c.
+// T = {da + db t + r : t in [-T/2, tl], |r| <= rho}, da = a_s - a_c, db = b_s - b_c, rho = e_s + e_c; m1
+// rotated by R_k+1 lies in T too, so rotated by R_k it lies in R_1^-1 T (R_1 the rotation by OMEGA_PT),
+// within ROT_W |m1| of T.  Every point of the segment is a convex combination of its ends.
 //
-//   level 1 (FP32): x^T U^-1 x >= |x|^2 / lam(U) >= (dist(0, axis segment) - rho')^2 / (lam_s + lam_c),
-//                   rho' = rho + ROT_W (|da| + 4 |db| + rho) covering the rotated segment ends
-//   level 2 (FP64): for every vector y, x^T U^-1 x >= 2 y.x - y^T U y (the quadratic form's dual), and a
-//                   linear function's minimum over the hull of T and R_1^-1 T is its minimum over either,
-//                   so min >= 2 (min(min_t y.(da + db t), min_t (R_1 y).(da + db t)) - rho |y|) - y^T U y;
-//                   y along U^-1 x* with x* the U^-1-closest point of the axis segment, scaled optimally,
-//                   makes the bound tight (l2_pass), compared with explicit bounds of its rounding.
-//   fine level 2:   the same dual bound with the tile's covariances interpolated in time (l2f_pass,
-//                   TileRecF): one U for all points inflates the narrow axes of an ellipsoid that turns
-//                   or shears over the tile, while (1 - w) F + w L + eta I, with the slack E of the
-//                   interpolation folded into F and L along the long axes, follows it; the bound becomes
-//                   a quadratic in the time parameter, checked over the tile without divisions.
+//   level 1 (FP32): x^T M_k^-1 x >= |x|^2 / lam >= (dist(0, axis segment) - rho')^2 / (lam_s + lam_c),
+//                   rho' = rho + ROT_W (|da| + 4 |db| + rho) covering the rotated segment ends (l1_pass)
+//   level 2 (FP32, explicit rounding bounds): for every vector z, (z.x)^2 <= (x^T M^-1 x)(z^T M z); z.x is
+//                   bounded below over the hull of T and R_1^-1 T by linear functions of the time
+//                   parameter, and z^T M_k z above by the linear interpolation of the two objects' records,
+//                   so the tile is dropped when a quadratic in the time parameter stays positive over the
+//                   tile (l2f_pass); z = U^-1 x* with x* the U^-1-closest point of the axis segment.  One U
+//                   for all points (l2_pass, FP64, kept as the tests' reference) inflates the narrow axes
+//                   of an ellipsoid that turns or shears over the tile; the interpolation follows it.
 //
 // A tile that fails either test holds no point with chi2 < chi2_max (1 + CUT_REL), so it can hold neither
 // the minimum of a kept pair nor change which pairs are kept.  CUT_REL covers the reference's own FP64

@@ -121,3 +121,20 @@ extern "C" int cc_scan2(const double *strk, int ns, int snpts, const double *ctr
     delete[] s1; delete[] q1; delete[] sf; delete[] qf;
     return ntl;
 }
+
+// The records of tile j (T = 8) or coarse tile j (T = 16) of one object track, for the tests of their
+// defining bounds: out = the 24 floats of TileRecF then lam (TileRec1).  Returns the tile's first point k0
+// and writes its last point kp.
+extern "C" int cc_recf(const double *trk, int n, int j, int coarse, double R2, float *out, int *kp_out) {
+    TileRec1 r1;
+    TileRec2 r2;
+    TileRecF rf;
+    if (coarse) tile_records<TILE2>(trk, n, j, R2, r1, r2, &rf);
+    else tile_records<TILE>(trk, n, j, R2, r1, r2, &rf);
+    const float *v = reinterpret_cast<const float *>(&rf);
+    for (int u = 0; u < 24; u++) out[u] = v[u];
+    out[24] = r1.lam;
+    const int T = coarse ? TILE2 : TILE, k0 = j * T;
+    *kp_out = (k0 + T < n ? k0 + T : n) - 1;
+    return k0;
+}

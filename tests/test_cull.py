@@ -134,3 +134,88 @@ def test_level2_rotation_constants(harness):
     """The level-2 test rotates its dual vector by one point's frame rotation with literal cos / sin."""
     harness.cc_rot_check.restype = C.c_double
     assert harness.cc_rot_check() < 1e-15
+
+
+def _sym(v):
+    """3x3 from (xx xy xz yy yz zz)."""
+    return np.array([[v[0], v[1], v[2]], [v[1], v[3], v[4]], [v[2], v[4], v[5]]], dtype=np.float64)
+
+
+def _rot(k):
+    th = -0.5 * (220.0 / 977.7922216807892) / 8.122 * k   # OMEGA_PT k (cull.h; km/s per kpc/Myr as physics.h)
+    c, s = np.cos(th), np.sin(th)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+@pytest.mark.parametrize("coarse", [0, 1])
+def test_interpolated_records_bound_every_point(harness, oracle, workload, coarse):
+    """Each tile record's defining bounds, checked point by point in FP64 (numpy): every rotated covariance
+    (plus R^2 I for a cluster) lies below (1 - w) F + w L + eta I, below lam I, and every rotated position
+    within e of the stored FP32 axis a + b (k - tc); ispan = 1 / (kp - k0)."""
+    harness.cc_recf.restype = C.c_int
+    harness.cc_recf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    cl = workload.make_clusters(300, 7)
+    st = workload.make_stars(6, 8)
+    n, off, pos, vel, S = oracle.lin_clusters(cl)
+    npts = int(np.ceil(float(cl["tmax_myr"].max()) / 0.5)) + 1
+    tn, toff, tpos, tvel, tS = oracle.lin_targets(st, float(cl["tmax_myr"].max()))
+    objs = [(t, int(k), 0.0) for t, k in zip(_aos(tn, toff, tpos, tvel, tS, npts), tn)]
+    R2 = (2.0 * cl["r50_pc"] / 1000.0) ** 2
+    objs += [(t, int(k), float(r)) for t, k, r in zip(_aos(n, off, pos, vel, S, npts)[:60], n[:60], R2[:60])]
+    T = 16 if coarse else 8
+    out = np.zeros(25, np.float32)
+    kp = np.zeros(1, np.int32)
+    worst_cov, worst_lam, worst_pos, tiles = np.inf, np.inf, np.inf, 0
+    for trk, nk, r2 in objs:
+        trk = np.ascontiguousarray(trk)
+        for j in range((nk + T - 1) // T):
+            k0 = harness.cc_recf(trk.ctypes.data, nk, j, coarse, r2, out.ctypes.data, kp.ctypes.data)
+            kpp = int(kp[0])
+            a, b = out[0:3].astype(np.float64), out[3:6].astype(np.float64)
+            F, L = _sym(out[6:12].astype(np.float64)), _sym(out[12:18].astype(np.float64))
+            e, eta, tp, tl, ispan, lam = (float(x) for x in (out[18], out[19], out[20], out[21], out[22], out[24]))
+            tc = k0 + 0.5 * T
+            assert tp == kpp - tc and tl == min(k0 + T, nk - 1) - tc
+            assert ispan == (np.float32(1.0 / (kpp - k0)) if kpp > k0 else 0.0)
+            for k in range(k0, kpp + 1):
+                R = _rot(k)
+                q = R @ _sym(trk[k, 6:12]) @ R.T + r2 * np.eye(3)
+                w = (k - k0) / (kpp - k0) if kpp > k0 else 0.0
+                scale = np.trace(q)
+                worst_cov = min(worst_cov, np.linalg.eigvalsh((1 - w) * F + w * L + eta * np.eye(3) - q).min() / scale)
+                worst_lam = min(worst_lam, (lam - np.linalg.eigvalsh(q).max()) / scale)
+            for k in range(k0, min(k0 + T, nk - 1) + 1):
+                p = _rot(k) @ trk[k, 0:3]
+                worst_pos = min(worst_pos, e - np.linalg.norm(p - (a + b * (k - tc))))
+            tiles += 1
+    assert tiles > 100
+    assert worst_cov >= -1e-12 and worst_lam >= -1e-12 and worst_pos >= 0.0, (worst_cov, worst_lam, worst_pos)
+
+
+@pytest.mark.parametrize("chi2_max", [9.0, 100.0])
+def test_two_level_scan_exact_at_other_cuts(harness, oracle, workload, chi2_max):
+    """The scan's two-level order drops no tile holding a point under cuts other than 25 (other stars)."""
+    harness.cc_scan2.restype = C.c_int
+    harness.cc_scan2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                 C.c_double, C.c_void_p, C.c_void_p]
+    cl = workload.make_clusters(4087, 0)
+    st = workload.make_stars(4, 11)
+    n, off, pos, vel, S = oracle.lin_clusters(cl)
+    npts = int(np.ceil(float(cl["tmax_myr"].max()) / 0.5)) + 1
+    tn, toff, tpos, tvel, tS = oracle.lin_targets(st, float(cl["tmax_myr"].max()))
+    ct = np.ascontiguousarray(_aos(n, off, pos, vel, S, npts))
+    stt = np.ascontiguousarray(_aos(tn, toff, tpos, tvel, tS, npts))
+    R2 = np.ascontiguousarray((2.0 * cl["r50_pc"] / 1000.0) ** 2)
+    cn = np.ascontiguousarray(n.astype(np.int32))
+    ntl = (npts + 7) // 8
+    p2 = np.zeros((len(st), len(cl), ntl), np.uint8)
+    counts = np.zeros(4, np.int64)
+    assert harness.cc_scan2(stt.ctypes.data, len(st), npts, ct.ctypes.data, cn.ctypes.data, R2.ctypes.data, len(cl),
+                            npts, chi2_max, p2.ctypes.data, counts.ctypes.data) == ntl
+    q = _point_chi2(stt, ct, cn, R2)
+    qt = np.full(p2.shape, np.inf)
+    for j in range(ntl):
+        qt[:, :, j] = q[:, :, 8 * j:8 * j + 8].min(-1)
+    assert (qt < chi2_max).sum() > 100             # not vacuous
+    assert qt[p2 == 0].min() >= chi2_max * (1 - 1e-9)
+    assert p2.sum() < 3 * (qt < chi2_max).sum()    # and the bound is not vacuous either

@@ -116,6 +116,10 @@ int phx_set_split(PhxCtx *ctx, int parts);
  * and their stage times.  No counterpart in the reference. */
 int phx_autotune(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms);
 
+/* Set those four launch grids directly (grids[i] <= 0 keeps the library's default for that stage), e.g. to
+ * apply an earlier phx_autotune result without timing again.  No counterpart in the reference. */
+int phx_set_grids(PhxCtx *ctx, const int *grids);
+
 /* Parity access to intermediate device results of the last plain pass / the cluster tables. */
 int phx_cluster_tracks(PhxCtx *ctx, int *n, double *pos, double *vel, double *S, int npts_cap);
 int phx_star_tracks(PhxCtx *ctx, int first, int count, double *trk, int npts_cap);

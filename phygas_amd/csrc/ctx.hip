@@ -384,13 +384,20 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         if (!c->part_stream[i]) PHX_CK(hipStreamCreateWithFlags(&c->part_stream[i], hipStreamNonBlocking));
         if (!c->ev_join[i]) PHX_CK(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
     }
-    PHX_CK(hipEventRecord(c->ev_fork, c->st));   // uploads and cluster tables first
+    // the draws of every sub-pass in one kernel before the fork: a sub-pass's tracks then start without
+    // waiting for its own draw kernel to find room beside the other sub-passes' tracks
+    if (c->ps0.ensure(48 * (size_t)ns) || c->sig.ensure(288 * (size_t)ns)) return 1;
+    if (launch_lin_init_targets(a.stars, a.seeds, ns, n_samp, c->P, c->ps0.as<double>(), c->sig.as<double>(), c->st))
+        return 1;
+    PHX_CK(hipEventRecord(c->ev_fork, c->st));   // uploads, cluster tables and draws first
     for (int i = 1; i < parts; i++) PHX_CK(hipStreamWaitEvent(c->part_stream[i], c->ev_fork, 0));
     for (int i = 0; i < parts; i++) {
         const int b = c->part_base[i];
         FastArgs ai = a;
         ai.ns = c->part_base[i + 1] - b;
         ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b;
+        ai.ps0_in = c->ps0.as<double>() + (size_t)b * 6;
+        ai.sig_in = c->sig.as<double>() + (size_t)b * 36;
         FastWork &W = part_fw(c, i);
         if (i > 0) {   // the launch settings of part 0
             W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;
@@ -479,7 +486,7 @@ extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, do
     const bool timing = c->timing;
     c->timing = true;
     struct Knob { int *v; const char *stage; int opts[5]; };
-    Knob knobs[4] = {{&c->fw.scan_blocks, "scan", {1024, 2048, 4096, 8192, 0}},
+    Knob knobs[4] = {{&c->fw.scan_blocks, "scan", {512, 576, 768, 1024, 2048}},
                      {&c->fw.items_blocks, "items", {4096, 8192, 16384, 32768, 0}},
                      {&c->fw.reduce_blocks, "reduce", {1024, 2048, 4096, 8192, 0}},
                      {&c->fw.finish_blocks, "finish", {1024, 2048, 4096, 8192, 0}}};
@@ -506,6 +513,16 @@ extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, do
     }
     c->timing = timing;
     return rc;
+}
+
+extern "C" int phx_set_grids(PhxCtx *c, const int *grids) {
+    PHX_REQUIRE(grids != nullptr, "grids is null");
+    for (int i = 0; i < 4; i++) PHX_REQUIRE(grids[i] <= (1 << 20), "grid too large");
+    c->fw.scan_blocks = grids[0];
+    c->fw.items_blocks = grids[1];
+    c->fw.reduce_blocks = grids[2];
+    c->fw.finish_blocks = grids[3];
+    return 0;
 }
 
 extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {

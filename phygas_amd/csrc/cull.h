@@ -209,22 +209,25 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
     const int kp = (k0 + T < n ? k0 + T : n) - 1;     // last point
     const int m = kl - k0 + 1;
     const double tc = k0 + 0.5 * T;
-    double px[T + 1], py[T + 1], pz[T + 1];
-    double st = 0, stt = 0, sx = 0, sy = 0, sz = 0, stx = 0, sty = 0, stz = 0;
-#pragma unroll
-    for (int i = 0; i <= T; i++) {
-        const int k = k0 + (i < m ? i : m - 1);
+    // rotated position of point k, evaluated again where it is needed rather than kept in registers
+    auto rpos = [&](int k, double &x, double &y, double &z) {
         const double *p = trk + (size_t)k * 12;
         double c, s;
         rot_at(cs, k, c, s);
-        px[i] = c * p[0] - s * p[1];
-        py[i] = s * p[0] + c * p[1];
-        pz[i] = p[2];
+        x = c * p[0] - s * p[1];
+        y = s * p[0] + c * p[1];
+        z = p[2];
+    };
+    double st = 0, stt = 0, sx = 0, sy = 0, sz = 0, stx = 0, sty = 0, stz = 0;
+#pragma unroll
+    for (int i = 0; i <= T; i++) {
         if (i < m) {
-            const double t = k - tc;
+            double x, y, z;
+            rpos(k0 + i, x, y, z);
+            const double t = (k0 + i) - tc;
             st += t; stt += t * t;
-            sx += px[i]; sy += py[i]; sz += pz[i];
-            stx += t * px[i]; sty += t * py[i]; stz += t * pz[i];
+            sx += x; sy += y; sz += z;
+            stx += t * x; sty += t * y; stz += t * z;
         }
     }
     double b[3] = {0.0, 0.0, 0.0};
@@ -240,11 +243,13 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
 #pragma unroll
     for (int i = 0; i <= T; i++) {
         if (i < m) {
+            double x, y, z;
+            rpos(k0 + i, x, y, z);
             const double t = (k0 + i) - tc;
-            const double rx = px[i] - (a[0] + b[0] * t), ry = py[i] - (a[1] + b[1] * t), rz = pz[i] - (a[2] + b[2] * t);
+            const double rx = x - (a[0] + b[0] * t), ry = y - (a[1] + b[1] * t), rz = z - (a[2] + b[2] * t);
             e64 = fmax(e64, sqrt(rx * rx + ry * ry + rz * rz));
-            const double fx = px[i] - ((double)af[0] + (double)bf[0] * t), fy = py[i] - ((double)af[1] + (double)bf[1] * t),
-                         fz = pz[i] - ((double)af[2] + (double)bf[2] * t);
+            const double fx = x - ((double)af[0] + (double)bf[0] * t), fy = y - ((double)af[1] + (double)bf[1] * t),
+                         fz = z - ((double)af[2] + (double)bf[2] * t);
             e32 = fmax(e32, sqrt(fx * fx + fy * fy + fz * fz));
         }
     }
@@ -260,30 +265,37 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
     kp_out = kp;
 }
 
-// The interpolated record of a tile from the rotated covariances qs of its points k0 .. kp (qs[k - k0]), the
-// axis in r2 and trmax >= every trace.  D_k = q_k - ((1 - w_k) q_k0 + w_k q_kp) is the interpolation's error;
-// over a tile the covariance follows a smooth curve, so the D_k are nearly multiples of one matrix whose
-// positive part lies along the ellipsoid's long axes.  E = that positive part of the D_k with the largest
-// eigenvalue bound goes into F and L, and eta bounds what is left (E is approximate; eta is an upper bound).
-// (first = qs[0], last = qs[kp - k0]: passed apart so that every array index is a constant.)
-template <int T = TILE>
-PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const double *last, int k0, int kp, int n,
+// The interpolated record of a tile from the rotated covariances q_k of its points k0 .. kp (q_at(k - k0, q)
+// writes q_k), the axis in r2 and trmax >= every trace.  D_k = q_k - ((1 - w_k) q_k0 + w_k q_kp) is the
+// interpolation's error; over a tile the covariance follows a smooth curve, so the D_k are nearly multiples
+// of one matrix whose positive part lies along the ellipsoid's long axes.  E = that positive part of the D_k
+// with the largest eigenvalue bound goes into F and L, and eta bounds what is left (E is approximate; eta is
+// an upper bound).  (first = q_k0, last = q_kp.)  q_k and D_k are evaluated again in each loop rather than
+// kept, so a lane holds a few of them at a time.
+template <int T = TILE, class QAt>
+PHX_HDF void tile_record_f(QAt q_at, const double *first, const double *last, int k0, int kp, int n,
                            const TileRec2 &r2, double R2, double trmax, TileRecF &rf, TileRec1 &r1) {
     const int kl = k0 + T < n - 1 ? k0 + T : n - 1;
     const double tc = k0 + 0.5 * T;
     const int np = kp - k0;
-    double D[T][6], Db[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, best = -1.0;
+    auto d_at = [&](int i, double *D) {   // D_{k0 + i}
+        const double w = np > 0 ? (double)i / (double)np : 0.0;
+        double q[6];
+        q_at(i, q);
+#pragma unroll
+        for (int u = 0; u < 6; u++) D[u] = q[u] - ((1.0 - w) * first[u] + w * last[u]);
+    };
+    double Db[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, best = -1.0;
 #pragma unroll
     for (int i = 0; i < T; i++) {
         if (i <= np) {
-            const double w = np > 0 ? (double)i / (double)np : 0.0;
-#pragma unroll
-            for (int u = 0; u < 6; u++) D[i][u] = qs[i][u] - ((1.0 - w) * first[u] + w * last[u]);
-            const double lm = sym3_lmax_bound(D[i]);
+            double D[6];
+            d_at(i, D);
+            const double lm = sym3_lmax_bound(D);
             if (lm > best) {
                 best = lm;
 #pragma unroll
-                for (int u = 0; u < 6; u++) Db[u] = D[i][u];
+                for (int u = 0; u < 6; u++) Db[u] = D[u];
             }
         }
     }
@@ -294,8 +306,9 @@ PHX_HDF void tile_record_f(const double (*qs)[6], const double *first, const dou
     for (int i = 0; i < T; i++) {
         if (i <= np) {
             double d[6];
+            d_at(i, d);
 #pragma unroll
-            for (int u = 0; u < 6; u++) d[u] = D[i][u] - E[u];
+            for (int u = 0; u < 6; u++) d[u] -= E[u];
             eta = fmax(eta, sym3_lmax_bound(d));
         }
     }
@@ -344,28 +357,38 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     int k0, kp;
     tile_line<T>(trk, n, j, r1, r2, k0, kp, cs);
     r2.R2 = R2;
-    double qs[T][6], first[6], ref[6], trmax = 0.0;   // rotated covariances of the tile's points
+    // rotated covariance of point k0 + i
+    auto q_at = [&](int i, double *q) {
+        double c, s;
+        rot_at(cs, k0 + i, c, s);
+        rot_cov(trk + (size_t)(k0 + i) * 12 + 6, c, s, q);
+    };
+    double first[6], ref[6], trmax = 0.0;
+    if (rf) {
+#pragma unroll
+        for (int i = 0; i < T; i++) {
+            if (k0 + i <= kp) {
+                double q[6];
+                q_at(i, q);
+                trmax = fmax(trmax, q[0] + q[3] + q[5]);
+            }
+        }
+        q_at(0, first);
+        q_at(kp - k0, ref);
+        tile_record_f<T>(q_at, first, ref, k0, kp, n, r2, R2, trmax, *rf, r1);
+        return;
+    }
+    double qs[T][6];   // rotated covariances of the tile's points
 #pragma unroll
     for (int i = 0; i < T; i++) {
-        const int k = k0 + i;
-        if (k <= kp) {
-            double c, s;
-            rot_at(cs, k, c, s);
-            rot_cov(trk + (size_t)k * 12 + 6, c, s, qs[i]);
+        if (k0 + i <= kp) {
+            q_at(i, qs[i]);
             trmax = fmax(trmax, qs[i][0] + qs[i][3] + qs[i][5]);
         }
     }
-    {
-        double c, s;
-        rot_at(cs, kp, c, s);
-        rot_cov(trk + (size_t)kp * 12 + 6, c, s, ref);
-    }
+    q_at(kp - k0, ref);
 #pragma unroll
     for (int u = 0; u < 6; u++) first[u] = qs[0][u];
-    if (rf) {
-        tile_record_f<T>(qs, first, ref, k0, kp, n, r2, R2, trmax, *rf, r1);
-        return;
-    }
     // One Loewner bound of the rotated covariances of the tile's points: ref + eta1 I (ref = the last point's)
     // or, for tiles of more than one point, sum + eta2 I (sum = ref + the positive part of the first point's
     // minus ref), the one with the larger tr(U^-1); level 1 the smaller eigenvalue bound.

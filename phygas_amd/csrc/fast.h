@@ -10,6 +10,7 @@ struct FastTables {
     void *mem[16] = {};
     size_t bytes[16] = {};
     int nc = 0, npts = 0, ntiles = 0, nc_pad = 0;
+    int n_cu = 256;                    // compute units of the device (the scan's default grid follows it)
     const int *d_cn = nullptr;         // resident cluster point counts and kernel radii (owned by the context)
     const double *d_R2 = nullptr;
 };
@@ -20,7 +21,7 @@ struct FastWork {
     size_t bytes[24] = {};
     long cand_cap = 0, kept_cap = 0, row_cap = 0, item_cap = 0, surv_cap = 0;
     long last_cand = 0, last_items = 0, last_l1 = 0, last_l2 = 0;
-    // launch grids (0: the defaults, which phx_autotune chose on the 40,000-star x 4,087-cluster unit);
+    // launch grids (0: the defaults, chosen on the 40,000-star x 4,087-cluster unit run as two sub-passes);
     // phx_autotune picks them by timing the stages on the loaded batch
     int scan_blocks = 0, items_blocks = 0, reduce_blocks = 0, finish_blocks = 0;
     // hipGraph of the pass's launches, replayed while sizes, parameters and buffers stay the same
@@ -42,6 +43,9 @@ struct FastArgs {
     const double *csoa;
     const int *cn;
     const double *R2, *r50, *rt;
+    // ps0 / Sigma0 of the batch's stars when already drawn (the sub-passes of a split pass share one draw
+    // kernel); null: the pass draws them into its own buffers
+    const double *ps0_in = nullptr, *sig_in = nullptr;
 };
 
 typedef void (*StageMark)(void *ctx, const char *name);

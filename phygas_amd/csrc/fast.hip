@@ -126,10 +126,11 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
     const double az = P.disk_a + zz * izeta;
     const double iD = rsq64(R2 + az * az);
     const double fd = -P.disk_gm * (iD * iD * iD);
-    const double ir = r2 > 0.0 ? rsq64(r2) : 0.0;
+    const double ir0 = rsq64(r2 > 0.0 ? r2 : 1.0), ir = r2 > 0.0 ? ir0 : 0.0;   // selects, no branch
     const double r = r2 * ir;
     const double irs = r > 1e-12 ? ir : 1e12;
-    const double xh = (r > 1e-12 ? (r < P.halo_cut ? r : P.halo_cut) : (1e-12 < P.halo_cut ? 1e-12 : P.halo_cut)) * inv_ha;
+    const double rl = r > 1e-12 ? r : 1e-12;
+    const double xh = (rl < P.halo_cut ? rl : P.halo_cut) * inv_ha;
     const double lx = log2(xh);
     const double gm = P.halo_gm * exp2(P.halo_gamma * lx) * rcp64(1.0 + exp2((P.halo_gamma - 1.0) * lx));
     const double fh = -gm * (irs * irs * irs);
@@ -181,8 +182,17 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
                     for (int c = 0; c < 3; c++) dx[i][c] += dv[i][c] * dt;
                 a = accel_fast64(P, inv_ha, x);
                 const float xf = (float)x.x, yf = (float)x.y, zf = (float)x.z;
+                // one branch per step for the rare terms outside the series' range or the halo cutoff
+                float tmax = 0.0f, r2max = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 3; i++) da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+                for (int i = 0; i < 3; i++)
+                    da[i] = dforce32_t<true>(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2], tmax,
+                                             r2max);
+                if (!(tmax < 1e-2f && r2max < Q.halo_cut2)) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+                        da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+                }
                 v = v + a * h;
 #pragma unroll
                 for (int i = 0; i < 3; i++) {
@@ -1191,6 +1201,12 @@ int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, cons
     T.nc = nc;
     T.npts = npts;
     T.ntiles = n_tiles(npts);
+    {
+        int dev = 0, ncu = 0;
+        PHX_CK(hipGetDevice(&dev));
+        PHX_CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ncu > 0) T.n_cu = ncu;
+    }
     const long ntiled = (long)nc * T.ntiles * TREC;
     if (ensure(T.mem[T_CTRK], T.bytes[T_CTRK], sizeof(double) * 12 * (size_t)nc * npts) ||
         ensure(T.mem[T_CTILED], T.bytes[T_CTILED], sizeof(double) * (size_t)ntiled) ||
@@ -1289,7 +1305,12 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
     TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
-    if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) return 1;
+    if (a.ps0_in) {
+        ps0 = const_cast<double *>(a.ps0_in);
+        sig = const_cast<double *>(a.sig_in);
+    } else if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) {
+        return 1;
+    }
     if (mark) mark(mark_ctx, "draws");
     if (a.exact_tracks) {   // the plain FP64 tracks: rows identical to the plain form's
         if (launch_lin_plain(ps0, sig, nullptr, trk, a.P, PH_LIN_EPS_POS, PH_LIN_EPS_VEL, -STEP_MYR / PH_LIN_SUB,
@@ -1314,7 +1335,11 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
     auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
-    hipLaunchKernelGGL(scan, dim3(W.scan_blocks > 0 ? W.scan_blocks : 2048), dim3(256), 0, st,
+    // default grid: 2.25 blocks (9 wavefronts) per CU, below the 3 per CU that fit (VGPRs): the scan's tasks
+    // are taken dynamically, so a grid the device holds at once leaves no blocks waiting, and the room left
+    // on each SIMD lets the other sub-pass's kernels run beside the scan instead of queueing behind it
+    const int scan_grid = W.scan_blocks > 0 ? W.scan_blocks : T.n_cu * 9 / 4;
+    hipLaunchKernelGGL(scan, dim3(scan_grid), dim3(256), 0, st,
                        (const TileRec1 *)T.mem[T_CREC1C], (const TileRecF *)T.mem[T_CRECFC],
                        (const TileRecF *)T.mem[T_CRECF], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
@@ -1393,7 +1418,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     uint64_t bits[2];
     memcpy(bits, dv, sizeof bits);
     mix(bits[0]); mix(bits[1]);
-    const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt};
+    const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt, a.ps0_in, a.sig_in};
     for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
     for (int i = 0; i < 24; i++) mix((uint64_t)(uintptr_t)W.mem[i]);
     for (int i = 0; i < 16; i++) mix((uint64_t)(uintptr_t)T.mem[i]);

@@ -98,26 +98,43 @@ PHX_HDF F3 force32(const PotF &Q, float x, float y, float z) {
     return a;
 }
 
-// (1 + t)^e - 1 for |t| small, relative accuracy ~ FP32 of the result; outside the series range the
-// direct form (loses relative precision, only reached far from any orbit this path integrates).
+// (1 + t)^e - 1 for |t| < 1e-2 as a series, relative accuracy ~ FP32 of the result.
+PHX_HDF float pw1_series(float t, float e) {
+    const float c2 = e * (e - 1.0f) * 0.5f, c3 = c2 * (e - 2.0f) * (1.0f / 3.0f), c4 = c3 * (e - 3.0f) * 0.25f,
+                c5 = c4 * (e - 4.0f) * 0.2f;
+    return t * f_fma(t, f_fma(t, f_fma(t, f_fma(t, c5, c4), c3), c2), e);
+}
+
+// (1 + t)^e - 1: the series for |t| < 1e-2; outside that range the direct form (loses relative precision,
+// only reached far from any orbit this path integrates).
 PHX_HDF float pw1(float t, float e) {
-    if (fabsf(t) < 1e-2f) {
-        const float c2 = e * (e - 1.0f) * 0.5f, c3 = c2 * (e - 2.0f) * (1.0f / 3.0f), c4 = c3 * (e - 3.0f) * 0.25f,
-                    c5 = c4 * (e - 4.0f) * 0.2f;
-        return t * f_fma(t, f_fma(t, f_fma(t, f_fma(t, c5, c4), c3), c2), e);
-    }
+    if (fabsf(t) < 1e-2f) return pw1_series(t, e);
     return f_pow(1.0f + t, e) - 1.0f;
 }
 
-// a(p + d) - a(p), every rounding relative to the difference.
-PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy, float dz) {
+// a(p + d) - a(p), every rounding relative to the difference.  Fast = true takes the common case without
+// branches: every (1 + t)^e - 1 through its series and the halo inside its cutoff at both positions; it
+// raises tmax to the largest |t| of its series and r2max to the larger squared radius.  The caller checks
+// tmax < 1e-2 and r2max < halo_cut2 (a NaN fails the check) and otherwise takes the Fast = false form, which
+// branches per term.  Where the check holds both forms run the same operations, so their results are
+// identical.
+template <bool Fast>
+PHX_HDF F3 dforce32_t(const PotF &Q, float x, float y, float z, float dx, float dy, float dz, float &tmax,
+                      float &r2max) {
+    auto pw = [&](float t, float e) {
+        if (Fast) {
+            tmax = fmaxf(tmax, fabsf(t));
+            return pw1_series(t, e);
+        }
+        return pw1(t, e);
+    };
     const float R2 = f_fma(x, x, y * y), r2 = f_fma(z, z, R2);
     const float dR2 = f_fma(dx, 2.0f * x + dx, dy * (2.0f * y + dy));
     const float dr2 = f_fma(dz, 2.0f * z + dz, dR2);
     // bulge: -GM p u^-3/2, u = r^2 + b^2
     const float u0 = r2 + Q.b2;
     const float iu0 = f_rsqrt(u0), g0 = iu0 * iu0 * iu0;
-    const float dg = g0 * pw1(f_div(dr2, u0), -1.5f);
+    const float dg = g0 * pw(f_div(dr2, u0), -1.5f);
     const float g1 = g0 + dg;
     // disk: -GM (x, y, z (a + zeta) / zeta) D^-3, D^2 = R^2 + (a + zeta)^2, zeta = sqrt(z^2 + b^2)
     const float zeta0 = f_sqrt_hw(f_fma(z, z, Q.db2)), zeta1 = f_sqrt_hw(f_fma(z + dz, z + dz, Q.db2));
@@ -126,7 +143,7 @@ PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy
     const float D20 = f_fma(A0, A0, R2);
     const float dD2 = f_fma(dzeta, 2.0f * A0 + dzeta, dR2);
     const float iD0 = f_rsqrt(D20), h0 = iD0 * iD0 * iD0;
-    const float dh = h0 * pw1(f_div(dD2, D20), -1.5f);
+    const float dh = h0 * pw(f_div(dD2, D20), -1.5f);
     const float h1 = h0 + dh;
     // z (1 + a / zeta): difference of z / zeta is (dz zeta0 - z dzeta) / (zeta0 zeta1)
     const float q0 = z * (1.0f + f_div(Q.disk_a, zeta0));
@@ -134,17 +151,18 @@ PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy
     // halo
     float f0, df;
     const float r21 = r2 + dr2;
-    if (r2 < Q.halo_cut2 && r21 < Q.halo_cut2) {
+    if (Fast) r2max = fmaxf(r2max, fmaxf(r2, r21));
+    if (Fast || (r2 < Q.halo_cut2 && r21 < Q.halo_cut2)) {
         const float t = f_div(dr2, r2);
         const float N0 = f_pow(r2, Q.halo_p1), E0 = Q.halo_k2 * f_pow(r2, Q.halo_p2);
-        const float dN = N0 * pw1(t, Q.halo_p1), dE = E0 * pw1(t, Q.halo_p2);
+        const float dN = N0 * pw(t, Q.halo_p1), dE = E0 * pw(t, Q.halo_p2);
         const float Dn0 = 1.0f + E0, Dn1 = Dn0 + dE;
         f0 = f_div(Q.halo_k1 * N0, Dn0);
         df = f_div(Q.halo_k1 * (dN * Dn0 - N0 * dE), Dn0 * Dn1);
     } else if (r2 >= Q.halo_cut2 && r21 >= Q.halo_cut2) {
         const float ir = f_rsqrt(r2);
         f0 = Q.halo_gmcut * ir * ir * ir;
-        df = f0 * pw1(f_div(dr2, r2), -1.5f);
+        df = f0 * pw(f_div(dr2, r2), -1.5f);
     } else {   // straddles the cutoff: plain difference
         const float ir0 = f_rsqrt(r2), ir1 = f_rsqrt(r21);
         f0 = r2 < Q.halo_cut2 ? Q.halo_k1 * f_pow(r2, Q.halo_p1) / (1.0f + Q.halo_k2 * f_pow(r2, Q.halo_p2))
@@ -166,6 +184,11 @@ PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy
     d.y = -(f_fma(dkxy, y, kxy1 * dy));
     d.z = -(f_fma(dkz, z, kz1 * dz) + Q.disk_gm * f_fma(dh, q0, h1 * dq));
     return d;
+}
+
+PHX_HDF F3 dforce32(const PotF &Q, float x, float y, float z, float dx, float dy, float dz) {
+    float tmax = 0.0f, r2max = 0.0f;
+    return dforce32_t<false>(Q, x, y, z, dx, dy, dz, tmax, r2max);
 }
 
 // float-float accumulation: (hi, lo) += v

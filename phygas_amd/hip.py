@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
+        "phx_set_grids": (_I, [_P, _P]),
         "phx_prefilter": (_I, [_P, _P, _I, _P, _I, _P]),
     }
     for name, (res, args) in sig.items():
@@ -236,6 +237,11 @@ class Engine:
                "phx_autotune")
         return {f"{k}_blocks": int(g) for k, g in zip(("scan", "items", "reduce", "finish"), grids)} | \
             {f"{k}_ms": float(t) for k, t in zip(("scan", "items", "reduce", "finish"), ms)}
+
+    def set_grids(self, scan: int = 0, items: int = 0, reduce: int = 0, finish: int = 0):
+        """Launch grids of the fast pass's scan, items, reduce and finish (0: the library's default)."""
+        g = np.array([scan, items, reduce, finish], np.int32)
+        _check(self._L.phx_set_grids(self.h, g.ctypes.data), "phx_set_grids")
 
     def sync(self):
         _check(self._L.phx_sync(self.h), "phx_sync")

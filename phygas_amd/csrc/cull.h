@@ -511,6 +511,9 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
     return !(L > 0.0 && L * L > chi2c * (zuz + err_u) * (1.0 + 1e-12));
 }
 
+// Level 2 is evaluated without early exits: every condition is computed and the results combined, so the
+// lanes of a wavefront (different pairs) stay on one instruction stream.
+//
 // FP32 check that g(t) = l(t)^2 - c V(t) > 0 and l(t) > 0 over [ta, tb] for the true l(t) >= l0 + l1 t - el
 // and V(t) <= v0 + v1 t + ev, el = 1e-6 (|l0| + |l1| tm) and ev = 1e-6 (|v0| + |v1| tm) (tm >= |t|) covering
 // the rounding of l0, l1, v0, v1 and of their evaluation: l is linear, so its lower bound positive at both
@@ -519,19 +522,16 @@ PHX_HDF bool l2_pass(const TileRec2 &s, const TileRec2 &c, double t1, double chi
 PHX_HDF bool quad_clear(float l0, float l1, float v0, float v1, float c, float ta, float tb, float tm) {
     const float el = 1e-6f * f_fma(fabsf(l1), tm, fabsf(l0)), ev = 1e-6f * f_fma(fabsf(v1), tm, fabsf(v0));
     const float la = f_fma(l1, ta, l0) - el, lb = f_fma(l1, tb, l0) - el;
-    if (!(la > 0.0f) || !(lb > 0.0f)) return false;
     const float va = c * (f_fma(v1, ta, v0) + ev), vb = c * (f_fma(v1, tb, v0) + ev);
-    if (!(f_fma(la, la, -va) > 1e-5f * f_fma(la, la, va))) return false;
-    if (!(f_fma(lb, lb, -vb) > 1e-5f * f_fma(lb, lb, vb))) return false;
+    bool ok = (la > 0.0f) & (lb > 0.0f);
+    ok &= f_fma(la, la, -va) > 1e-5f * f_fma(la, la, va);
+    ok &= f_fma(lb, lb, -vb) > 1e-5f * f_fma(lb, lb, vb);
     const float A = l1 * l1, B = f_fma(2.0f * l0, l1, -c * v1), C = f_fma(l0, l0, -c * v0);
-    if (A > 0.0f && -B > 2.0f * A * ta && -B < 2.0f * A * tb) {   // the vertex -B / 2A inside
-        // its value C - B^2 / 4A moves by at most 2 |l| el + c ev under the perturbations; 4A times that and
-        // the rounding of the discriminant are below the margin
-        const float lm = f_fma(fabsf(l1), tm, fabsf(l0)), vm = f_fma(fabsf(v1), tm, fabsf(v0));
-        const float m = 1e-4f * f_fma(4.0f * A, f_fma(lm, lm, c * vm), B * B);
-        return f_fma(4.0f * A, C, -B * B) > m;
-    }
-    return true;
+    const bool inside = (A > 0.0f) & (-B > 2.0f * A * ta) & (-B < 2.0f * A * tb);
+    const float lm = f_fma(fabsf(l1), tm, fabsf(l0)), vm = f_fma(fabsf(v1), tm, fabsf(v0));
+    const float m = 1e-4f * f_fma(4.0f * A, f_fma(lm, lm, c * vm), B * B);
+    const bool vok = f_fma(4.0f * A, C, -B * B) > m;
+    return ok & (!inside | vok);
 }
 
 // Fine level 2 with the covariances interpolated in time (TileRecF), in FP32 with explicit error bounds.
@@ -557,7 +557,7 @@ PHX_HDF bool l2f_pass(const TileRecF &s, const TileRecF &c, float chi2c) {
     const float trL = (s.L[0] + s.L[3] + s.L[5]) + (c.L[0] + c.L[3] + c.L[5]);
     // every M_k has trace <= max(trF, trL) + 3 eta and smallest eigenvalue >= R^2: condition number bounded
     // (the bound taken 1% below 1e9 for the rounding)
-    if (!(f_fma(3.0f, eta, fmaxf(trF, trL)) < 0.99e9f * (s.R2 + c.R2))) return true;
+    const bool guard = !(f_fma(3.0f, eta, fmaxf(trF, trL)) < 0.99e9f * (s.R2 + c.R2));
     U[0] += eta; U[3] += eta; U[5] += eta;
     const float t0 = -0.5f * T, tp = fminf(s.tp, c.tp), tl = fminf(s.tl, c.tl), tm = 0.5f * T;
     auto dot = [](float a0, float a1, float a2, float b0, float b1, float b2) {
@@ -576,7 +576,7 @@ PHX_HDF bool l2f_pass(const TileRecF &s, const TileRecF &c, float chi2c) {
     const float x0 = f_fma(wb0, ts, wa0), x1 = f_fma(wb1, ts, wa1), x2 = f_fma(wb2, ts, wa2);
     float z2 = x2 * i2, z1 = f_fma(-l21, z2, x1) * i1, z0 = f_fma(-l20, z2, f_fma(-l10, z1, x0)) * i0;
     const float n2 = dot(z0, z1, z2, z0, z1, z2);
-    if (!(n2 > 1e-30f && n2 < 1e30f)) return true;
+    const bool nbad = !((n2 > 1e-30f) & (n2 < 1e30f));
     const float r = f_rsqrt(n2);
     z0 *= r; z1 *= r; z2 *= r;   // |z| <= 1 + 1e-6
     const float zr0 = f_fma((float)ROT_C, z0, -(float)ROT_S * z1), zr1 = f_fma((float)ROT_S, z0, (float)ROT_C * z1);
@@ -595,9 +595,9 @@ PHX_HDF bool l2f_pass(const TileRecF &s, const TileRecF &c, float chi2c) {
     const float el = 1e-5f * (f_fma(tm, nb1, na1) + rho);   // the differences' and dot products' rounding
     const float la0 = dot(z0, z1, z2, da0, da1, da2) - (rho + el), la1 = dot(z0, z1, z2, db0, db1, db2);
     const float lb0 = dot(zr0, zr1, z2, da0, da1, da2) - (rho + el), lb1 = dot(zr0, zr1, z2, db0, db1, db2);
-    const bool clear = quad_clear(la0, la1, v0, v1, chi2c, t0, tp, tm) &&
-                       (tl < t0 + 1.0f || quad_clear(lb0, lb1, v0 - v1, v1, chi2c, t0 + 1.0f, tl, tm));   // no far end: kl = k0
-    return !clear;
+    const bool c1 = quad_clear(la0, la1, v0, v1, chi2c, t0, tp, tm);
+    const bool c2 = (tl < t0 + 1.0f) | quad_clear(lb0, lb1, v0 - v1, v1, chi2c, t0 + 1.0f, tl, tm);   // no far end: kl = k0
+    return guard | nbad | !(c1 & c2);
 }
 
 }  // namespace phx

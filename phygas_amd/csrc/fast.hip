@@ -1335,10 +1335,10 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
     auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
-    // default grid: 2.25 blocks (9 wavefronts) per CU, below the 3 per CU that fit (VGPRs): the scan's tasks
+    // default grid: 2 blocks (2 wavefronts per SIMD) per CU, below the 3 that fit (VGPRs): the scan's tasks
     // are taken dynamically, so a grid the device holds at once leaves no blocks waiting, and the room left
     // on each SIMD lets the other sub-pass's kernels run beside the scan instead of queueing behind it
-    const int scan_grid = W.scan_blocks > 0 ? W.scan_blocks : T.n_cu * 9 / 4;
+    const int scan_grid = W.scan_blocks > 0 ? W.scan_blocks : T.n_cu * 2;
     hipLaunchKernelGGL(scan, dim3(scan_grid), dim3(256), 0, st,
                        (const TileRec1 *)T.mem[T_CREC1C], (const TileRecF *)T.mem[T_CRECFC],
                        (const TileRecF *)T.mem[T_CRECF], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],

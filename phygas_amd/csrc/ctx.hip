@@ -398,6 +398,9 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b;
         ai.ps0_in = c->ps0.as<double>() + (size_t)b * 6;
         ai.sig_in = c->sig.as<double>() + (size_t)b * 36;
+        // the last sub-pass's tracks with one lane per star: the sub-passes' tracks kernels then take fewer
+        // wavefronts than the device has SIMDs, so none shares a SIMD with another and none waits for one
+        ai.track_lanes = i == parts - 1 ? 1 : 2;
         FastWork &W = part_fw(c, i);
         if (i > 0) {   // the launch settings of part 0
             W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;

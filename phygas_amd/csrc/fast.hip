@@ -138,14 +138,19 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
 }
 
 // ------------------------------------------------------------------------------------------- tracks
-// Two lanes per star; lane q propagates offsets 3q .. 3q+2 (x, y, z for q = 0; vx, vy, vz for q = 1) and
-// both lanes run the same FP64 nominal.  Track layout as the plain path: [star][npts][12] FP64.
+// LPS lanes per star.  LPS = 2: lane q propagates offsets 3q .. 3q+2 (x, y, z for q = 0; vx, vy, vz for
+// q = 1) and both lanes run the same FP64 nominal.  LPS = 1: one lane propagates all six offsets and runs
+// the nominal once (fewer operations per star, a longer chain per lane).  Both give the same bits: every
+// offset's arithmetic is the same.  Track layout as the plain path: [star][npts][12] FP64.
+template <int LPS>
 __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
                                                   double *__restrict__ trk, Pot P, PotF Q, int npts, int n,
                                                   double *__restrict__ tiled, int ntl) {
 #pragma clang fp contract(fast)
+    static_assert(LPS == 1 || LPS == 2, "one or two lanes per star");
+    constexpr int NO = 6 / LPS;   // offsets per lane
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = gid >> 1, q = gid & 1;
+    const int s = gid / LPS, q = LPS == 2 ? (gid & 1) : 0;
     const bool live = s < n;
     const double eps = PH_LIN_EPS_POS, inv_eps = 1.0 / PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
     V3 x = v3(0, 0, 0), v = v3(0, 0, 0);
@@ -154,30 +159,31 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
         x = v3(p[0], p[1], p[2]);
         v = v3(p[3], p[4], p[5]);
     }
-    double dx[3][3], dv[3][3];
+    double dx[NO][3], dv[NO][3];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < NO; i++)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            dx[i][c] = (q == 0 && c == i) ? eps : 0.0;
-            dv[i][c] = (q == 1 && c == i) ? eps : 0.0;
+            const int o = 3 * q + i;   // offset o: position component o (o < 3), velocity component o - 3
+            dx[i][c] = (o < 3 && c == o) ? eps : 0.0;
+            dv[i][c] = (o >= 3 && c == o - 3) ? eps : 0.0;
         }
     const double inv_ha = 1.0 / P.halo_a;
     V3 a = accel_fast64(P, inv_ha, x);
-    F3 da[3];
+    F3 da[NO];
 #pragma unroll
-    for (int i = 0; i < 3; i++) da[i] = dforce32(Q, (float)x.x, (float)x.y, (float)x.z, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+    for (int i = 0; i < NO; i++) da[i] = dforce32(Q, (float)x.x, (float)x.y, (float)x.z, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
     for (int k = 0; k < npts; k++) {
         if (k > 0) {
             for (int it = 0; it < PH_LIN_SUB; it++) {
                 v = v + a * h;
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
+                for (int i = 0; i < NO; i++) {
                     dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
                 }
                 x = x + v * dt;
 #pragma unroll
-                for (int i = 0; i < 3; i++)
+                for (int i = 0; i < NO; i++)
 #pragma unroll
                     for (int c = 0; c < 3; c++) dx[i][c] += dv[i][c] * dt;
                 a = accel_fast64(P, inv_ha, x);
@@ -185,32 +191,39 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
                 // one branch per step for the rare terms outside the series' range or the halo cutoff
                 float tmax = 0.0f, r2max = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 3; i++)
+                for (int i = 0; i < NO; i++)
                     da[i] = dforce32_t<true>(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2], tmax,
                                              r2max);
                 if (!(tmax < 1e-2f && r2max < Q.halo_cut2)) {
 #pragma unroll
-                    for (int i = 0; i < 3; i++)
+                    for (int i = 0; i < NO; i++)
                         da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
                 }
                 v = v + a * h;
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
+                for (int i = 0; i < NO; i++) {
                     dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
                 }
             }
         }
-        // A[r][3q+i] = dx[i][r] / eps; lane 0 receives lane 1's three columns
+        // A[r][o] = dx[o][r] / eps; with two lanes, lane 0 receives lane 1's three columns
         double A[3][6];
+        if (LPS == 2) {
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+            for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const double mine = dx[i][r] * inv_eps;
-                const double other = __shfl_xor(mine, 1, 64);
-                A[r][i] = q == 0 ? mine : other;
-                A[r][3 + i] = q == 0 ? other : mine;
-            }
+                for (int r = 0; r < 3; r++) {
+                    const double mine = dx[i % NO][r] * inv_eps;
+                    const double other = __shfl_xor(mine, 1, 64);
+                    A[r][i] = q == 0 ? mine : other;
+                    A[r][3 + i] = q == 0 ? other : mine;
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) A[r][i] = dx[i % NO][r] * inv_eps;
+        }
         if (!live || q != 0) continue;
         const double *sg = sig + (size_t)s * 36;
         double AS[3][6];
@@ -1319,8 +1332,12 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
         hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts,
                            ns, npts, ntl, (double *)W.mem[W_TILED]);
     } else {   // writes the tiled copy too
-        hipLaunchKernelGGL(k_lin_fast, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
-                           make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
+        if (a.track_lanes == 1)
+            hipLaunchKernelGGL(k_lin_fast<1>, dim3(blocks_for((long)ns, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
+                               make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
+        else
+            hipLaunchKernelGGL(k_lin_fast<2>, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk,
+                               a.P, make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
     }
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
@@ -1412,7 +1429,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     uint64_t key = 1469598103934665603ULL;
     auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };
     mix((uint64_t)a.ns); mix((uint64_t)a.nc); mix((uint64_t)a.npts); mix((uint64_t)a.n_samp);
-    mix((uint64_t)a.exact_tracks); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
+    mix((uint64_t)a.exact_tracks); mix((uint64_t)a.track_lanes); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
     mix((uint64_t)W.reduce_blocks); mix((uint64_t)W.finish_blocks);
     double dv[2] = {a.chi2_max, a.p_min};
     uint64_t bits[2];

@@ -175,11 +175,15 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
     for (int i = 0; i < NO; i++) da[i] = dforce32(Q, (float)x.x, (float)x.y, (float)x.z, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
     for (int k = 0; k < npts; k++) {
         if (k > 0) {
+            // kick-drift-kick sub-steps with each sub-step's second half kick merged into the next one's
+            // first (the same acceleration): K(h) D K(dt) D ... K(dt) D K(h)
             for (int it = 0; it < PH_LIN_SUB; it++) {
-                v = v + a * h;
+                const double kick = it == 0 ? h : dt;
+                v = v + a * kick;
 #pragma unroll
                 for (int i = 0; i < NO; i++) {
-                    dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
+                    dv[i][0] += (double)da[i].x * kick; dv[i][1] += (double)da[i].y * kick;
+                    dv[i][2] += (double)da[i].z * kick;
                 }
                 x = x + v * dt;
 #pragma unroll
@@ -199,11 +203,11 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
                     for (int i = 0; i < NO; i++)
                         da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
                 }
-                v = v + a * h;
+            }
+            v = v + a * h;   // the last half kick: the velocities at the point
 #pragma unroll
-                for (int i = 0; i < NO; i++) {
-                    dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
-                }
+            for (int i = 0; i < NO; i++) {
+                dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
             }
         }
         // A[r][o] = dx[o][r] / eps; with two lanes, lane 0 receives lane 1's three columns

@@ -245,7 +245,7 @@ def main() -> int:
         "config": {"workload": "screen_20k_x_4087_with_shuffled_control", "stars": len(stars), "control_stars":
                    len(control), "clusters": len(clusters), "pairs_per_step": pairs, "chi2_max": chi2_max,
                    "n_samp": 64, "r_kernel": 2.0, "p_min": p_min, "form": result["form"],
-                   "sub_passes": args.split if args.split > 0 else 2, "cluster_tables_s": tables_s},
+                   "sub_passes": args.split if args.split > 0 else 3, "cluster_tables_s": tables_s},
         "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "pass_stats": stats, "autotune": tune,
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -105,9 +105,9 @@ void phx_set_timing(PhxCtx *ctx, int on);
  * same (a pass with timing on runs eagerly, its stage events between the kernels). */
 void phx_set_graphs(PhxCtx *ctx, int on);
 int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
-/* A fast pass over a batch of at least 4,096 stars runs as `parts` sub-passes (1..4; default 2) over
- * contiguous star ranges, each on its own stream, so that one's latency-bound tracks overlap another's scan;
- * rows are the same.  A pass with timing on runs whole.  No counterpart in the reference (its screen is one
+/* A fast pass over a batch of at least 4,096 stars runs as `parts` sub-passes (1..4; default 3) over
+ * contiguous star ranges, each on its own stream, so that one's latency-bound tracks overlap another's scan
+ * (the last sub-pass's tracks take one lane per star); rows are the same.  A pass with timing on runs whole.  No counterpart in the reference (its screen is one
  * host loop over the targets, src/phygas_screen.c:48-67). */
 int phx_set_split(PhxCtx *ctx, int parts);
 

@@ -73,7 +73,7 @@ struct PhxCtx {
     double last_chi2_max = 25.0, last_p_min = 0.0;
     int last_n_samp = 64;
     FastWork fw;       // fast-form work buffers (part 0 of a split pass)
-    int split = 2;     // sub-passes of a large fast pass (phx_set_split)
+    int split = 3;     // sub-passes of a large fast pass (phx_set_split)
     int nparts = 1;    // sub-passes of the last fast pass, over stars [part_base[i], part_base[i + 1])
     int part_base[MAX_SPLIT + 1] = {};
     long part_rows[MAX_SPLIT] = {};

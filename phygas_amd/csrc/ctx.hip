@@ -377,7 +377,12 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     const int ns = a.ns;
     int parts = c->timing || ns < 2 * SPLIT_MIN ? 1 : std::min(c->split, MAX_SPLIT);
     c->nparts = parts;
-    for (int i = 0; i <= parts; i++) c->part_base[i] = (int)((long)ns * i / parts);
+    // the last sub-pass (tracks with one lane per star, the longest) takes 40% of the stars and the others
+    // share the rest equally: measured on the 40,000-star unit, 16.9-17.0 ms per pass against 17.1-17.4 with
+    // equal thirds (the pass time moves with how the sub-passes' kernels meet, not smoothly with the sizes)
+    const int last = parts > 1 ? (int)((long)ns * 2 / 5) : ns;
+    for (int i = 0; i < parts; i++) c->part_base[i] = parts > 1 ? (int)((long)(ns - last) * i / (parts - 1)) : 0;
+    c->part_base[parts] = ns;
     if (parts == 1) return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
     if (!c->ev_fork) PHX_CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     for (int i = 1; i < parts; i++) {

@@ -132,7 +132,8 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
     const double rl = r > 1e-12 ? r : 1e-12;
     const double xh = (rl < P.halo_cut ? rl : P.halo_cut) * inv_ha;
     const double lx = log2(xh);
-    const double gm = P.halo_gm * exp2(P.halo_gamma * lx) * rcp64(1.0 + exp2((P.halo_gamma - 1.0) * lx));
+    const double e1 = exp2((P.halo_gamma - 1.0) * lx);   // xh^(gamma - 1); xh^gamma = xh e1
+    const double gm = P.halo_gm * (xh * e1) * rcp64(1.0 + e1);
     const double fh = -gm * (irs * irs * irs);
     return v3((fb + fd + fh) * p.x, (fb + fd + fh) * p.y, (fb + fh) * p.z + fd * p.z * (az * izeta));
 }

@@ -266,7 +266,7 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
 }
 
 // The interpolated record of a tile from the rotated covariances q_k of its points k0 .. kp (q_at(k - k0, q)
-// writes q_k), the axis in r2 and trmax >= every trace.  D_k = q_k - ((1 - w_k) q_k0 + w_k q_kp) is the
+// writes q_k) and the axis in r2.  D_k = q_k - ((1 - w_k) q_k0 + w_k q_kp) is the
 // interpolation's error; over a tile the covariance follows a smooth curve, so the D_k are nearly multiples
 // of one matrix whose positive part lies along the ellipsoid's long axes.  E = that positive part of the D_k
 // with the largest eigenvalue bound goes into F and L, and eta bounds what is left (E is approximate; eta is
@@ -274,14 +274,16 @@ PHX_HDF void tile_line(const double *trk, int n, int j, TileRec1 &r1, TileRec2 &
 // kept, so a lane holds a few of them at a time.
 template <int T = TILE, class QAt>
 PHX_HDF void tile_record_f(QAt q_at, const double *first, const double *last, int k0, int kp, int n,
-                           const TileRec2 &r2, double R2, double trmax, TileRecF &rf, TileRec1 &r1) {
+                           const TileRec2 &r2, double R2, TileRecF &rf, TileRec1 &r1) {
     const int kl = k0 + T < n - 1 ? k0 + T : n - 1;
     const double tc = k0 + 0.5 * T;
     const int np = kp - k0;
-    auto d_at = [&](int i, double *D) {   // D_{k0 + i}
+    double trmax = 0.0;   // >= every point's trace (taken in the first loop)
+    auto d_at = [&](int i, double *D, bool tr) {   // D_{k0 + i}
         const double w = np > 0 ? (double)i / (double)np : 0.0;
         double q[6];
         q_at(i, q);
+        if (tr) trmax = fmax(trmax, q[0] + q[3] + q[5]);
 #pragma unroll
         for (int u = 0; u < 6; u++) D[u] = q[u] - ((1.0 - w) * first[u] + w * last[u]);
     };
@@ -290,7 +292,7 @@ PHX_HDF void tile_record_f(QAt q_at, const double *first, const double *last, in
     for (int i = 0; i < T; i++) {
         if (i <= np) {
             double D[6];
-            d_at(i, D);
+            d_at(i, D, true);
             const double lm = sym3_lmax_bound(D);
             if (lm > best) {
                 best = lm;
@@ -306,7 +308,7 @@ PHX_HDF void tile_record_f(QAt q_at, const double *first, const double *last, in
     for (int i = 0; i < T; i++) {
         if (i <= np) {
             double d[6];
-            d_at(i, d);
+            d_at(i, d, false);
 #pragma unroll
             for (int u = 0; u < 6; u++) d[u] -= E[u];
             eta = fmax(eta, sym3_lmax_bound(d));
@@ -365,17 +367,9 @@ PHX_HDF void tile_records(const double *trk, int n, int j, double R2, TileRec1 &
     };
     double first[6], ref[6], trmax = 0.0;
     if (rf) {
-#pragma unroll
-        for (int i = 0; i < T; i++) {
-            if (k0 + i <= kp) {
-                double q[6];
-                q_at(i, q);
-                trmax = fmax(trmax, q[0] + q[3] + q[5]);
-            }
-        }
         q_at(0, first);
         q_at(kp - k0, ref);
-        tile_record_f<T>(q_at, first, ref, k0, kp, n, r2, R2, trmax, *rf, r1);
+        tile_record_f<T>(q_at, first, ref, k0, kp, n, r2, R2, *rf, r1);
         return;
     }
     double qs[T][6];   // rotated covariances of the tile's points

@@ -111,9 +111,10 @@ int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
  * host loop over the targets, src/phygas_screen.c:48-67). */
 int phx_set_split(PhxCtx *ctx, int parts);
 
-/* Launch grids of the fast pass's tile scan, item evaluation, kept-pair reduction and finish, chosen by
- * timing each on the loaded batch (results do not depend on them); grids[4] and ms[4] receive the choices
- * and their stage times.  No counterpart in the reference. */
+/* Launch grids of the fast pass's tile scan, item evaluation, kept-pair reduction and finish, chosen one
+ * after another by timing whole passes over the loaded batch as the library runs them (results do not
+ * depend on them); grids[4] and ms[4] receive the choices and their pass times.  No counterpart in the
+ * reference. */
 int phx_autotune(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms);
 
 /* Set those four launch grids directly (grids[i] <= 0 keeps the library's default for that stage), e.g. to

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -491,27 +492,27 @@ extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, do
     if (phx_screen(c, form, chi2_max, n_samp, p_min)) return 1;
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;   // sizes the buffers
+    // each option is timed on whole passes as the library runs them (sub-passes overlapping), best of two
+    struct Knob { int *v; int opts[5]; };
+    Knob knobs[4] = {{&c->fw.scan_blocks, {512, 576, 768, 1024, 2048}},
+                     {&c->fw.items_blocks, {4096, 8192, 16384, 32768, 65536}},
+                     {&c->fw.reduce_blocks, {1024, 2048, 4096, 8192, 0}},
+                     {&c->fw.finish_blocks, {1024, 2048, 4096, 8192, 0}}};
     const bool timing = c->timing;
-    c->timing = true;
-    struct Knob { int *v; const char *stage; int opts[5]; };
-    Knob knobs[4] = {{&c->fw.scan_blocks, "scan", {512, 576, 768, 1024, 2048}},
-                     {&c->fw.items_blocks, "items", {4096, 8192, 16384, 32768, 0}},
-                     {&c->fw.reduce_blocks, "reduce", {1024, 2048, 4096, 8192, 0}},
-                     {&c->fw.finish_blocks, "finish", {1024, 2048, 4096, 8192, 0}}};
+    c->timing = false;
     int rc = 0;
     for (int q = 0; q < 4 && rc == 0; q++) {
         int best = *knobs[q].v;
         double best_ms = 1e300;
         for (int o = 0; o < 5 && knobs[q].opts[o] > 0; o++) {
             *knobs[q].v = knobs[q].opts[o];
-            double t = 0.0;
-            for (int rep = 0; rep < 2 && rc == 0; rep++) {   // best of two passes
+            double t = 1e300;
+            for (int rep = 0; rep < 3 && rc == 0; rep++) {   // the first pass with new grids is not counted
+                const auto t0 = std::chrono::steady_clock::now();
                 if ((rc = phx_screen(c, form, chi2_max, n_samp, p_min)) != 0) break;
-                double st[MAX_STAGES];
-                const char *nm[MAX_STAGES];
-                const int n = phx_stage_times(c, st, nm, MAX_STAGES);
-                for (int i = 0; i < n; i++)
-                    if (strcmp(nm[i], knobs[q].stage) == 0) t = rep == 0 ? st[i] : std::min(t, st[i]);
+                if (hipStreamSynchronize(c->st) != hipSuccess) { rc = 1; break; }
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (rep > 0) t = std::min(t, ms);
             }
             if (rc == 0 && t < best_ms) { best_ms = t; best = knobs[q].opts[o]; }
         }

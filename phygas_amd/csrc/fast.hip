@@ -1375,13 +1375,13 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
                        cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "expand");
-    hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 32768), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 65536), dim3(256), 0, st,
                        (const Item *)W.mem[W_ITEMS], cnt + C_ITEMS, W.item_cap,
                        (const double *)W.mem[W_TILED], (const double *)T.mem[T_CTILED], a.cn, a.R2, npts, ntl,
                        (IRes *)W.mem[W_IRES]);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "items");
-    hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 8192), dim3(256), 0, st, cand,
+    hipLaunchKernelGGL(k_reduce, dim3(W.reduce_blocks > 0 ? W.reduce_blocks : 1024), dim3(256), 0, st, cand,
                        cnt + C_CAND, W.cand_cap, (const long *)W.mem[W_ISTART], (const IRes *)W.mem[W_IRES],
                        cnt + C_ITEMS, W.item_cap, a.chi2_max, (Kept *)W.mem[W_KSEG], cnt + C_KSEG0, W.kept_cap);
     PHX_CK(hipGetLastError());
@@ -1399,7 +1399,7 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     hipLaunchKernelGGL(k_fin_setup, dim3(kgrid), dim3(256), 0, st, kept, cnt + C_KEPT, W.kept_cap, trk, ctrk, a.cn,
                        a.r50, a.rt, a.cl, a.prefix, npts, n50_min, fscr, fdraw, fcnt);
     PHX_CK(hipGetLastError());
-    hipLaunchKernelGGL(k_finish_fast, dim3(W.finish_blocks > 0 ? W.finish_blocks : 4096), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_finish_fast, dim3(W.finish_blocks > 0 ? W.finish_blocks : 8192), dim3(256), 0, st,
                        (const FinScr *)fscr, cnt + C_KEPT, W.kept_cap, (uint64_t *)W.mem[W_SURV], cnt + C_SURV,
                        W.surv_cap);
     PHX_CK(hipGetLastError());

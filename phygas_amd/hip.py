@@ -230,7 +230,8 @@ class Engine:
         return dict(zip(("l1_tests", "l2_tests", "candidates", "items", "kept", "rows"), (int(x) for x in st)))
 
     def autotune(self, form: int = FAST, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> dict:
-        """Pick the fast pass's launch grids by timing its stages on the loaded batch; returns the choices."""
+        """Pick the fast pass's launch grids by timing whole passes on the loaded batch; returns the choices and
+        the pass time of each."""
         grids = np.zeros(4, np.int32)
         ms = np.zeros(4)
         _check(self._L.phx_autotune(self.h, form, chi2_max, n_samp, p_min, grids.ctypes.data, ms.ctypes.data),

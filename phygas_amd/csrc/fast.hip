@@ -5,8 +5,9 @@
 //   1. draws         ps0 / Sigma0 per star (the plain FP64 kernel; 64 draws per star)
 //   2. k_lin_fast    tracks: FP64 nominal orbit, the six neighbour offsets of the finite-difference STM
 //                    propagated with the FP32 difference force (force32.h) and FP64 accumulators,
-//                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each.  Writes the
-//                    AoS tracks and their tiled copy.
+//                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each (one lane
+//                    with all six in the last sub-pass of a split pass).  Writes the AoS tracks and their
+//                    tiled copy.
 //   3. k_star_recs   per (star, 8-point tile) and per (star, 16-point coarse tile): the tile records of
 //                    cull.h (axis line, tube radius and Loewner covariance bounds in the co-rotating frame:
 //                    one bound's largest eigenvalue for level 1, the interpolation between the tile's ends

@@ -129,6 +129,10 @@ int phx_npts(PhxCtx *ctx);
 /* Largest error of the FP32 Box-Muller estimate the fast finish screens draws with, over n uniform pairs of
  * splitmix64(seed) (the screen keeps draws within 1e-3 of a counting interval). */
 int phx_check_finish_screen(PhxCtx *ctx, uint64_t seed, long n, double *worst);
+/* The correctly rounded pow / x^1.5 / hypot / sin / cos that the device uses in the plain path's force and
+ * coordinates, over n arguments: fn 0 pow(x, y), 1 x^1.5, 2 hypot(x, y), 3 sin x, 4 cos x (y unused for 1, 3,
+ * 4); on_device 0 evaluates the host compilation of the same source (no device calls). */
+int phx_check_crmath(int fn, int on_device, const double *x, const double *y, long n, double *out);
 
 /* Monte Carlo clones already drawn (ps[n][12] = target pos/vel, cluster pos/vel): closest approach (kpc),
  * its time (Myr ago) and relative speed (kpc/Myr), per clone (reference: src/phygas.c:200-216). */

@@ -16,6 +16,21 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef ORC_ULP_SHIM
+/* Test-only build (liboracle_ulp.so): pow, sin, cos and hypot of the C library moved by one unit in the last
+ * place in a chosen share of calls (orc_ulp_set; a hash of the arguments decides, so a given argument always
+ * gets the same result, as under another libm).  It measures how far last-bit differences in the
+ * transcendentals move the tracks and the chi2 minima. */
+double orc_ulp_pow(double x, double y);
+double orc_ulp_sin(double x);
+double orc_ulp_cos(double x);
+double orc_ulp_hypot(double x, double y);
+#define pow orc_ulp_pow
+#define sin orc_ulp_sin
+#define cos orc_ulp_cos
+#define hypot orc_ulp_hypot
+#endif
+
 /* ---- units and constants (reference: src/units.h:14,27-28; src/coords.c:18,31-35,58-65,71) ---- */
 #define KMS_PER_KPCMYR 977.7922216807892
 #define G_INTERNAL (4.30091e-6 / (KMS_PER_KPCMYR * KMS_PER_KPCMYR))
@@ -953,3 +968,34 @@ void orc_prefilter_mask(const PhTarget *tg, int nt, const PhCluster *cl, int nc,
     for (int i = 0; i < nc; i++) ph_track_free(&ct[i]);
     free(ct);
 }
+
+#ifdef ORC_ULP_SHIM
+#undef pow
+#undef sin
+#undef cos
+#undef hypot
+static unsigned g_ulp_mask;
+static double g_ulp_rate;
+
+/* mask bits: 1 pow, 2 sin, 4 cos, 8 hypot; rate: the share of calls moved */
+void orc_ulp_set(unsigned mask, double rate) {
+    g_ulp_mask = mask;
+    g_ulp_rate = rate;
+}
+
+static double ulp_move(unsigned bit, double v, double x, double y) {
+    if (!(g_ulp_mask & bit)) return v;
+    uint64_t bx, by;
+    memcpy(&bx, &x, 8);
+    memcpy(&by, &y, 8);
+    uint64_t h = bx * 0x9E3779B97F4A7C15ULL ^ (by + bit) * 0xBF58476D1CE4E5B9ULL;
+    h = (h ^ (h >> 31)) * 0x94D049BB133111EBULL;
+    h ^= h >> 29;
+    if ((double)(h >> 11) * (1.0 / 9007199254740992.0) < g_ulp_rate) v = nextafter(v, (h & 1) ? INFINITY : -INFINITY);
+    return v;
+}
+double orc_ulp_pow(double x, double y) { return ulp_move(1, pow(x, y), x, y); }
+double orc_ulp_sin(double x) { return ulp_move(2, sin(x), x, 0.0); }
+double orc_ulp_cos(double x) { return ulp_move(4, cos(x), x, 0.0); }
+double orc_ulp_hypot(double x, double y) { return ulp_move(8, hypot(x, y), x, y); }
+#endif

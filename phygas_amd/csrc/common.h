@@ -38,11 +38,16 @@ const char *last_error();
 inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 // ---- plain FP64 launchers (plain.hip) -----------------------------------------------------------
-// Draws of every object (nominal start state ps0[n][6] and the 6x6 covariance sig[n][36] of n_samp draws).
-int launch_lin_init_targets(const PhTarget *d_t, const uint64_t *d_seed, int n, int n_samp, const Pot &P,
-                            double *d_ps0, double *d_sig, hipStream_t st);
-int launch_lin_init_clusters(const PhCluster *d_c, const uint64_t *d_seed, int n, int n_samp, const Pot &P,
-                             double *d_ps0, double *d_sig, hipStream_t st);
+// Draws of every object: the start state ps0[n][6] (a copy of the host's nominal nom[n][6]) and the 6x6
+// covariance sig[n][36] of n_samp draws about it.
+int launch_lin_init_targets(const PhTarget *d_t, const uint64_t *d_seed, const double *d_nom, int n, int n_samp,
+                            const Pot &P, double *d_ps0, double *d_sig, hipStream_t st);
+int launch_lin_init_clusters(const PhCluster *d_c, const uint64_t *d_seed, const double *d_nom, int n, int n_samp,
+                             const Pot &P, double *d_ps0, double *d_sig, hipStream_t st);
+// Nominal start states of n records on the host with the C library's transcendentals, as the reference
+// draws them (reference: src/phygas.c:50-82 with rng == NULL): out[n][6]; zeros where every retry fails.
+void nominal_states(const Pot &P, const PhTarget *t, int n, double *out);
+void nominal_states(const Pot &P, const PhCluster *c, int n, double *out);
 // Nominal + 6 neighbour orbits, points [k0, k0+nk).  soa_nc == 0: track layout [obj][npts][12];
 // soa_nc > 0: cluster layout [point][12][soa_nc].  d_state (63 doubles/object) may be null when k0 == 0
 // and nk covers every point.

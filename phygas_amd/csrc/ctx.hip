@@ -59,14 +59,14 @@ struct PhxCtx {
     double r_kernel = 2.0;
     std::vector<int> cn_h;
     std::vector<PhCluster> cl_h;
-    DBuf cl, cseed, cps0, csig, csoa, cn, R2, r50, rt;
+    DBuf cl, cseed, cnom, cps0, csig, csoa, cn, R2, r50, rt;
     FastTables fast;   // fast-form tables (fast.hip), built on first use
     bool fast_ready = false;
     // stars
     std::vector<PhTarget> st_h;
     std::vector<int> group_h;
     bool stars_dirty = true;
-    DBuf sr, sseed, sprefix, sgroup;
+    DBuf sr, sseed, sprefix, sgroup, snom;
     // plain pass
     DBuf ps0, sig, trk, chi2, kb, sb, seltmp, idx, counts, fout, ridx, rows;
     long kept_cap = 0, row_cap = 0;
@@ -135,7 +135,7 @@ extern "C" void phx_destroy(PhxCtx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
-    DBuf *bufs[] = {&c->cl, &c->cseed, &c->cps0, &c->csig, &c->csoa, &c->cn, &c->R2, &c->r50, &c->rt,
+    DBuf *bufs[] = {&c->cl, &c->cseed, &c->cnom, &c->snom, &c->cps0, &c->csig, &c->csoa, &c->cn, &c->R2, &c->r50, &c->rt,
                     &c->sr, &c->sseed, &c->sprefix, &c->sgroup, &c->ps0, &c->sig, &c->trk, &c->chi2,
                     &c->kb, &c->sb, &c->seltmp, &c->idx, &c->counts, &c->fout, &c->ridx, &c->rows,
                     &c->pps, &c->psteps, &c->pdts, &c->pst, &c->pres, &c->poff, &c->pm, &c->pr50,
@@ -227,18 +227,22 @@ extern "C" int phx_load_clusters(PhxCtx *c, const PhCluster *cl, int nc, int n_s
         r50[i] = cl[i].r50_pc / 1000.0;
         rt[i] = cl[i].rtot_pc / 1000.0;
     }
-    if (c->cl.ensure(sizeof(PhCluster) * nc) || c->cseed.ensure(8 * nc) || c->cps0.ensure(48 * nc) ||
+    std::vector<double> nom((size_t)nc * 6);
+    nominal_states(c->P, cl, nc, nom.data());
+    if (c->cl.ensure(sizeof(PhCluster) * nc) || c->cseed.ensure(8 * nc) || c->cnom.ensure(48 * nc) ||
+        c->cps0.ensure(48 * nc) ||
         c->csig.ensure(288 * nc) || c->csoa.ensure(sizeof(double) * 12 * (size_t)c->npts * nc) ||
         c->cn.ensure(4 * nc) || c->R2.ensure(8 * nc) || c->r50.ensure(8 * nc) || c->rt.ensure(8 * nc))
         return 1;
     PHX_CK(hipMemcpyAsync(c->cl.p, cl, sizeof(PhCluster) * nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->cseed.p, seeds.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->cnom.p, nom.data(), 48 * (size_t)nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->cn.p, c->cn_h.data(), 4 * nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->R2.p, R2.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->r50.p, r50.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->rt.p, rt.data(), 8 * nc, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemsetAsync(c->csoa.p, 0, c->csoa.cap, c->st));
-    if (launch_lin_init_clusters(c->cl.as<PhCluster>(), c->cseed.as<uint64_t>(), nc, n_samp, c->P,
+    if (launch_lin_init_clusters(c->cl.as<PhCluster>(), c->cseed.as<uint64_t>(), c->cnom.as<double>(), nc, n_samp, c->P,
                                  c->cps0.as<double>(), c->csig.as<double>(), c->st))
         return 1;
     if (launch_lin_plain(c->cps0.as<double>(), c->csig.as<double>(), nullptr, c->csoa.as<double>(), c->P,
@@ -292,17 +296,20 @@ static int upload_stars(PhxCtx *c) {
     if (!c->stars_dirty) return 0;
     const int ns = (int)c->st_h.size();
     std::vector<uint64_t> seed(ns), prefix(ns);
+    std::vector<double> nom((size_t)ns * 6);
     for (int i = 0; i < ns; i++) {
         seed[i] = phx_seed("lin", c->st_h[i].id);
         prefix[i] = seed_prefix(c->st_h[i].id);
     }
+    nominal_states(c->P, c->st_h.data(), ns, nom.data());
     if (c->sr.ensure(sizeof(PhTarget) * ns) || c->sseed.ensure(8 * ns) || c->sprefix.ensure(8 * ns) ||
-        c->sgroup.ensure(4 * ns))
+        c->sgroup.ensure(4 * ns) || c->snom.ensure(48 * (size_t)ns))
         return 1;
     PHX_CK(hipMemcpyAsync(c->sr.p, c->st_h.data(), sizeof(PhTarget) * ns, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->sseed.p, seed.data(), 8 * ns, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->sprefix.p, prefix.data(), 8 * ns, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(c->sgroup.p, c->group_h.data(), 4 * ns, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(c->snom.p, nom.data(), 48 * (size_t)ns, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipStreamSynchronize(c->st));
     c->stars_dirty = false;
     return 0;
@@ -332,8 +339,8 @@ static int screen_plain(PhxCtx *c, double chi2_max, int n_samp, double p_min) {
         return 1;
     long *cnt = c->counts.as<long>();
     stage_begin(c);
-    if (launch_lin_init_targets(c->sr.as<PhTarget>(), c->sseed.as<uint64_t>(), ns, n_samp, c->P, c->ps0.as<double>(),
-                                c->sig.as<double>(), c->st))
+    if (launch_lin_init_targets(c->sr.as<PhTarget>(), c->sseed.as<uint64_t>(), c->snom.as<double>(), ns, n_samp, c->P,
+                                c->ps0.as<double>(), c->sig.as<double>(), c->st))
         return 1;
     stage_mark(c, "draws");
     if (launch_lin_plain(c->ps0.as<double>(), c->sig.as<double>(), nullptr, c->trk.as<double>(), c->P, PH_LIN_EPS_POS,
@@ -372,6 +379,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     a.P = c->P; a.ns = (int)c->st_h.size(); a.nc = c->nc; a.npts = c->npts; a.n_samp = n_samp;
     a.chi2_max = chi2_max; a.p_min = p_min; a.exact_tracks = form == PHX_FORM_FAST64;
     a.stars = c->sr.as<PhTarget>(); a.seeds = c->sseed.as<uint64_t>(); a.prefix = c->sprefix.as<uint64_t>();
+    a.nom = c->snom.as<double>();
     a.group = c->sgroup.as<int>(); a.cl = c->cl.as<PhCluster>(); a.csoa = c->csoa.as<double>();
     a.cn = c->cn.as<int>(); a.R2 = c->R2.as<double>(); a.r50 = c->r50.as<double>(); a.rt = c->rt.as<double>();
     auto mark = [](void *ctx, const char *name) { stage_mark((PhxCtx *)ctx, name); };
@@ -393,7 +401,8 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     // the draws of every sub-pass in one kernel before the fork: a sub-pass's tracks then start without
     // waiting for its own draw kernel to find room beside the other sub-passes' tracks
     if (c->ps0.ensure(48 * (size_t)ns) || c->sig.ensure(288 * (size_t)ns)) return 1;
-    if (launch_lin_init_targets(a.stars, a.seeds, ns, n_samp, c->P, c->ps0.as<double>(), c->sig.as<double>(), c->st))
+    if (launch_lin_init_targets(a.stars, a.seeds, a.nom, ns, n_samp, c->P, c->ps0.as<double>(), c->sig.as<double>(),
+                                c->st))
         return 1;
     PHX_CK(hipEventRecord(c->ev_fork, c->st));   // uploads, cluster tables and draws first
     for (int i = 1; i < parts; i++) PHX_CK(hipStreamWaitEvent(c->part_stream[i], c->ev_fork, 0));
@@ -401,7 +410,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         const int b = c->part_base[i];
         FastArgs ai = a;
         ai.ns = c->part_base[i + 1] - b;
-        ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b;
+        ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b; ai.nom += (size_t)b * 6;
         ai.ps0_in = c->ps0.as<double>() + (size_t)b * 6;
         ai.sig_in = c->sig.as<double>() + (size_t)b * 36;
         // the last sub-pass's tracks with one lane per star: the sub-passes' tracks kernels then take fewer

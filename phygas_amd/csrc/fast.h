@@ -38,6 +38,7 @@ struct FastArgs {
     int exact_tracks;                  // 1: the plain FP64 tracks instead of the FP32-offset tracks
     const PhTarget *stars;
     const uint64_t *seeds, *prefix;
+    const double *nom;                 // nominal start states [ns][6] (host-drawn)
     const int *group;
     const PhCluster *cl;
     const double *csoa;

@@ -1327,7 +1327,7 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     if (a.ps0_in) {
         ps0 = const_cast<double *>(a.ps0_in);
         sig = const_cast<double *>(a.sig_in);
-    } else if (launch_lin_init_targets(a.stars, a.seeds, ns, a.n_samp, a.P, ps0, sig, st)) {
+    } else if (launch_lin_init_targets(a.stars, a.seeds, a.nom, ns, a.n_samp, a.P, ps0, sig, st)) {
         return 1;
     }
     if (mark) mark(mark_ctx, "draws");
@@ -1441,7 +1441,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     uint64_t bits[2];
     memcpy(bits, dv, sizeof bits);
     mix(bits[0]); mix(bits[1]);
-    const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.group, a.cl, a.cn, a.R2, a.r50, a.rt, a.ps0_in, a.sig_in};
+    const void *ptrs[] = {a.stars, a.seeds, a.prefix, a.nom, a.group, a.cl, a.cn, a.R2, a.r50, a.rt, a.ps0_in, a.sig_in};
     for (const void *p : ptrs) mix((uint64_t)(uintptr_t)p);
     for (int i = 0; i < 24; i++) mix((uint64_t)(uintptr_t)W.mem[i]);
     for (int i = 0; i < 16; i++) mix((uint64_t)(uintptr_t)T.mem[i]);

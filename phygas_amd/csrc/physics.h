@@ -1,18 +1,43 @@
 // FP64 physics of the trace-back path, shared by host and gfx950 device code.
 //
 // Every function follows the reference operation by operation (cited per function) and the library is
-// built with -ffp-contract=off, so host code reproduces the reference's doubles exactly and device code
-// differs only where the device's libm (pow/log/exp/cos/sin/hypot) rounds differently from glibc.
+// built with -ffp-contract=off, so host code reproduces the reference's doubles exactly.  Device code
+// takes pow, x^1.5, hypot, sin and cos from crmath.h (correctly rounded), so it agrees with the host C
+// library wherever that rounds to nearest; it differs only where the C library itself misrounds (glibc:
+// about 0.1% of pow arguments, 0.15% of sin / cos, 0.5% of hypot) and in the Box-Muller draws' log / cos,
+// which only reach draw covariances and counts.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
+#include "crmath.h"
 #include "phygas_types.h"
 
 #define PHX_HD __host__ __device__ __forceinline__
 
 namespace phx {
+
+// The transcendentals of the force and the coordinate transform: the C library's on the host (the
+// reference's own calls), the correctly rounded ones on the device.  Overloaded on the compilation side,
+// so the shared functions below call the right one on each.
+__host__ __forceinline__ double pow15(double x) { return ::pow(x, 1.5); }
+__device__ __forceinline__ double pow15(double x) { return cr::pow15_cr(x); }
+__host__ __forceinline__ void pow_pair(double x, double y1, double y2, double &p1, double &p2) {
+    p1 = ::pow(x, y1);
+    p2 = ::pow(x, y2);
+}
+__device__ __forceinline__ void pow_pair(double x, double y1, double y2, double &p1, double &p2) {
+    cr::pow2_cr(x, y1, y2, p1, p2);
+}
+__host__ __forceinline__ double hypot_rn(double x, double y) { return ::hypot(x, y); }
+__device__ __forceinline__ double hypot_rn(double x, double y) { return cr::hypot_cr(x, y); }
+__host__ __forceinline__ void sin_cos(double x, double &s, double &c) {
+    s = ::sin(x);
+    c = ::cos(x);
+}
+__device__ __forceinline__ void sin_cos(double x, double &s, double &c) { cr::sincos_cr(x, s, c); }
 
 constexpr double KMS_PER_KPCMYR = 977.7922216807892;      // reference: src/units.h:14
 constexpr double PI = 3.14159265358979323846;
@@ -54,7 +79,7 @@ PHX_HD V3 accel(const Pot &P, V3 p) {
     double R = sqrt(R2);
     double r = norm(p);
     V3 a = v3(0.0, 0.0, 0.0);
-    double den = pow(r * r + P.bulge_b * P.bulge_b, 1.5);
+    double den = pow15(r * r + P.bulge_b * P.bulge_b);
     double fb = -P.bulge_gm / (den > 0 ? den : 1e-12);
     a = a + p * fb;
     double zeta = sqrt(p.z * p.z + P.disk_b * P.disk_b);
@@ -72,7 +97,9 @@ PHX_HD V3 accel(const Pot &P, V3 p) {
     double rs = r > 1e-12 ? r : 1e-12;
     double re = rs < P.halo_cut ? rs : P.halo_cut;
     double xh = re / P.halo_a;
-    double gm = P.halo_gm * pow(xh, P.halo_gamma) / (1.0 + pow(xh, P.halo_gamma - 1.0));
+    double pg, pg1;   // xh^gamma, xh^(gamma - 1)
+    pow_pair(xh, P.halo_gamma, P.halo_gamma - 1.0, pg, pg1);
+    double gm = P.halo_gm * pg / (1.0 + pg1);
     double fh = -gm / (rs * rs * rs);
     return a + p * fh;
 }
@@ -124,10 +151,12 @@ struct Astro { double ra, dec, plx, pmra, pmdec, rv; };
 PHX_HD void astro_to_gc(const Astro &a, V3 &pos, V3 &vel) {
     double ra = a.ra * (PI / 180.0), dec = a.dec * (PI / 180.0);
     double dk = 1.0 / a.plx, dp = dk * 1000.0;
-    double sr = sin(ra), cr = cos(ra), sd = sin(dec), cd = cos(dec);
-    V3 rh = v3(cd * cr, cd * sr, sd);
-    V3 eh = v3(-sr, cr, 0.0);
-    V3 nh = v3(-sd * cr, -sd * sr, cd);
+    double sa, ca, sd, cd;
+    sin_cos(ra, sa, ca);
+    sin_cos(dec, sd, cd);
+    V3 rh = v3(cd * ca, cd * sa, sd);
+    V3 eh = v3(-sa, ca, 0.0);
+    V3 nh = v3(-sd * ca, -sd * sa, cd);
     pos = gc_pos(rh * dk);
     const double k = (1.495978707e8 / 3.15576e7) * 1e-3;
     double vra = k * a.pmra * dp, vde = k * a.pmdec * dp;
@@ -137,7 +166,7 @@ PHX_HD void astro_to_gc(const Astro &a, V3 &pos, V3 &vel) {
 
 // Radial velocity of circular Galactic rotation at pos, seen from the Sun (reference: src/phygas.c:25-36).
 PHX_HD double rv_rotation(const Pot &P, V3 p) {
-    double R = hypot(p.x, p.y);
+    double R = hypot_rn(p.x, p.y);
     if (R <= 0) return 0.0;
     V3 a = accel(P, v3(p.x, p.y, 0.0));
     double ar = -(a.x * p.x + a.y * p.y) / R;

@@ -25,19 +25,20 @@ void set_error(const char *where, const char *what) {
 const char *last_error() { return g_err.c_str(); }
 
 // ---------------------------------------------------------------- draws -> (ps0, Sigma0)
-// One lane per object: nominal draw, then n_samp seeded draws accumulated into the 6x6 covariance about
-// the nominal, divided by the number of successful draws (reference: src/phygas.c:256-264, 274-284).
+// One lane per object: n_samp seeded draws accumulated into the 6x6 covariance about the nominal, divided by
+// the number of successful draws (reference: src/phygas.c:256-264, 274-284).  The nominal start state
+// nom[n][6] comes from the host (nominal_states), drawn with the C library as the reference draws it: the
+// finite-difference tracks amplify a one-ulp difference in a start state a hundred-thousand-fold.
 // The 21 unique sums are accumulated in draw order, which is the reference's order for every entry.
 template <class Rec, int IsCluster>
 __global__ __launch_bounds__(128) void k_lin_init(const Rec *__restrict__ recs, const uint64_t *__restrict__ seeds,
-                                                  int n, int n_samp, Pot P, double *__restrict__ ps0,
-                                                  double *__restrict__ sig) {
+                                                  const double *__restrict__ nom, int n, int n_samp, Pot P,
+                                                  double *__restrict__ ps0, double *__restrict__ sig) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Rec r = recs[i];
-    V3 np = v3(0, 0, 0), nv = v3(0, 0, 0);
-    if (IsCluster) sample_cluster(P, (const PhCluster &)r, nullptr, np, nv);
-    else sample_target(P, (const PhTarget &)r, nullptr, np, nv);
+    const double *q0 = nom + (size_t)i * 6;
+    const V3 np = v3(q0[0], q0[1], q0[2]), nv = v3(q0[3], q0[4], q0[5]);
     double acc[21];
 #pragma unroll
     for (int q = 0; q < 21; q++) acc[q] = 0.0;
@@ -71,19 +72,32 @@ __global__ __launch_bounds__(128) void k_lin_init(const Rec *__restrict__ recs, 
     p0[0] = np.x; p0[1] = np.y; p0[2] = np.z; p0[3] = nv.x; p0[4] = nv.y; p0[5] = nv.z;
 }
 
-int launch_lin_init_targets(const PhTarget *d_t, const uint64_t *d_seed, int n, int n_samp, const Pot &P,
-                            double *d_ps0, double *d_sig, hipStream_t st) {
+template <class Rec, int IsCluster>
+static void nominal_states_of(const Pot &P, const Rec *r, int n, double *out) {
+    for (int i = 0; i < n; i++) {
+        V3 p = v3(0, 0, 0), v = v3(0, 0, 0);
+        if (IsCluster) sample_cluster(P, (const PhCluster &)r[i], nullptr, p, v);
+        else sample_target(P, (const PhTarget &)r[i], nullptr, p, v);
+        double *o = out + (size_t)i * 6;
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = v.x; o[4] = v.y; o[5] = v.z;
+    }
+}
+void nominal_states(const Pot &P, const PhTarget *t, int n, double *out) { nominal_states_of<PhTarget, 0>(P, t, n, out); }
+void nominal_states(const Pot &P, const PhCluster *c, int n, double *out) { nominal_states_of<PhCluster, 1>(P, c, n, out); }
+
+int launch_lin_init_targets(const PhTarget *d_t, const uint64_t *d_seed, const double *d_nom, int n, int n_samp,
+                            const Pot &P, double *d_ps0, double *d_sig, hipStream_t st) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_lin_init<PhTarget, 0>), dim3(blocks_for(n, 128)), dim3(128), 0, st, d_t, d_seed, n, n_samp,
-                       P, d_ps0, d_sig);
+    hipLaunchKernelGGL((k_lin_init<PhTarget, 0>), dim3(blocks_for(n, 128)), dim3(128), 0, st, d_t, d_seed, d_nom, n,
+                       n_samp, P, d_ps0, d_sig);
     PHX_CK(hipGetLastError());
     return 0;
 }
 
-int launch_lin_init_clusters(const PhCluster *d_c, const uint64_t *d_seed, int n, int n_samp, const Pot &P,
-                             double *d_ps0, double *d_sig, hipStream_t st) {
+int launch_lin_init_clusters(const PhCluster *d_c, const uint64_t *d_seed, const double *d_nom, int n, int n_samp,
+                             const Pot &P, double *d_ps0, double *d_sig, hipStream_t st) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL((k_lin_init<PhCluster, 1>), dim3(blocks_for(n, 128)), dim3(128), 0, st, d_c, d_seed, n,
+    hipLaunchKernelGGL((k_lin_init<PhCluster, 1>), dim3(blocks_for(n, 128)), dim3(128), 0, st, d_c, d_seed, d_nom, n,
                        n_samp, P, d_ps0, d_sig);
     PHX_CK(hipGetLastError());
     return 0;
@@ -666,6 +680,24 @@ int launch_pair_plain(const double *d_ps, const int *d_steps, const double *d_dt
     return 0;
 }
 
+// ---------------------------------------------------------------- correctly rounded math, device side
+// The device evaluation of crmath.h over given arguments (fn: 0 pow(x, y), 1 x^1.5, 2 hypot(x, y), 3 sin x,
+// 4 cos x), for the parity check against the same source compiled for the host.
+__global__ __launch_bounds__(256) void k_crmath(int fn, const double *__restrict__ x, const double *__restrict__ y,
+                                                long n, double *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double s, c, v;
+        switch (fn) {
+        case 0: v = cr::pow_cr(x[i], y[i]); break;
+        case 1: v = cr::pow15_cr(x[i]); break;
+        case 2: v = cr::hypot_cr(x[i], y[i]); break;
+        case 3: cr::sincos_cr(x[i], s, c); v = s; break;
+        default: cr::sincos_cr(x[i], s, c); v = c; break;
+        }
+        out[i] = v;
+    }
+}
+
 }  // namespace phx
 
 // ================================================================ pg_* ABI (drop-in for the reference's
@@ -758,4 +790,38 @@ extern "C" int pg_pair(PgBuf ps, PgBuf steps, PgBuf dts, PgBuf st, PgBuf res, Pg
         return 1;
     PHX_CK(hipDeviceSynchronize());
     return 0;
+}
+
+static double crmath_eval(int fn, double x, double y) {
+    double s, c;
+    switch (fn) {
+    case 0: return cr::pow_cr(x, y);
+    case 1: return cr::pow15_cr(x);
+    case 2: return cr::hypot_cr(x, y);
+    case 3: cr::sincos_cr(x, s, c); return s;
+    default: cr::sincos_cr(x, s, c); return c;
+    }
+}
+
+extern "C" int phx_check_crmath(int fn, int on_device, const double *x, const double *y, long n, double *out) {
+    PHX_REQUIRE(fn >= 0 && fn <= 4 && n >= 0, "bad function or count");
+    if (n == 0) return 0;
+    if (!on_device) {   // the host compilation of the same source
+        for (long i = 0; i < n; i++) out[i] = crmath_eval(fn, x[i], y[i]);
+        return 0;
+    }
+    double *d = nullptr;
+    PHX_CK(hipMalloc(&d, 24 * (size_t)n));
+    int rc = 0;
+    if (hipMemcpy(d, x, 8 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, y, 8 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = 1;
+    } else {
+        hipLaunchKernelGGL(k_crmath, dim3(1024), dim3(256), 0, 0, fn, d, d + n, n, d + 2 * n);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(out, d + 2 * n, 8 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = 1;
+    }
+    (void)hipFree(d);
+    if (rc) set_error("phx_check_crmath", "device evaluation failed");
+    return rc;
 }

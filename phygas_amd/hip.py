@@ -73,6 +73,7 @@ def lib() -> C.CDLL:
         "phx_seed": (C.c_uint64, [C.c_char_p, C.c_char_p]),
         "phx_pair_draw": (_I, [_P, _P, _I, _D, C.c_uint64, _P, _P, _P]),
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
+        "phx_check_crmath": (_I, [_I, _I, _P, _P, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
@@ -114,6 +115,19 @@ def pair_draw(t, c, n_clones: int, dt_myr: float, seed_: int):
     dt = np.zeros(1)
     m = lib().phx_pair_draw(_p(t), _p(c), n_clones, dt_myr, seed_, _p(ps), _p(steps), _p(dt))
     return ps[:m], int(steps[0]), float(dt[0])
+
+
+CRMATH_FUNCS = {"pow": 0, "pow15": 1, "hypot": 2, "sin": 3, "cos": 4}
+
+
+def crmath(fn: str, x, y=None, device: bool = False) -> np.ndarray:
+    """The library's correctly rounded pow / x^1.5 / hypot / sin / cos (the device side of the plain path's
+    force and coordinates), evaluated on the device or by the host compilation of the same source."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.zeros_like(x)
+    _check(lib().phx_check_crmath(CRMATH_FUNCS[fn], int(device), _p(x), _p(y), x.size, _p(out)), "phx_check_crmath")
+    return out
 
 
 class WuOptions(C.Structure):

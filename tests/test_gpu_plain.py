@@ -1,12 +1,13 @@
 """GPU parity of the plain FP64 path against the CPU oracle (through the C ABI).
 
-Tolerances (north-star bar for the plain FP64 form): orbits within 1e-9 relative, p_r50 / p_rtot identical
-counts (|dp| <= 1e-4 means the same count of 512), Monte Carlo quantiles within 1e-9 relative here (the bar
-is 1e-3).  Device pow/log/cos/sin differ from glibc in the last bit, so these are tolerances, not bit
-identity.  The covariances come from a finite-difference state-transition matrix, A = (x_j - x_0) / 1e-5,
-which multiplies a one-ulp difference of an 8-kpc position by 1e5: covariance entries (compared relative
-to sqrt(S_ii S_jj)) and the Mahalanobis minima built on them carry that noise floor, COV_REL below; the
-measured maxima are written to gpurun_out/parity_plain.json."""
+Bar (the plain FP64 form's): orbits, covariances and Mahalanobis minima within 1e-9 relative, the minimum's
+point (kb, sb) identical, p_r50 / p_rtot identical counts, Monte Carlo quantiles within 1e-9 relative here
+(the Monte Carlo bar is 1e-3), row text identical.  The device takes pow, x^1.5, hypot, sin and cos correctly
+rounded (crmath.h) and every object's nominal start state from the host's C library, so it integrates the
+same doubles as the oracle except where glibc itself misrounds (about 0.1% of pow calls); the finite-
+difference state-transition matrix, A = (x_j - x_0) / 1e-5, amplifies such a difference 1e5-fold, which is why
+this is a tolerance and not bit identity (tests/test_libm_floor.py measures that floor on the CPU).  The
+measured maxima and the share of bit-identical values go to gpurun_out/parity_plain.json."""
 import json
 import os
 
@@ -22,7 +23,6 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INP = os.path.join(ROOT, "tests", "golden", "inputs")
 REL = 1e-9
-COV_REL = 1e-7
 REPORT = {}
 
 
@@ -72,7 +72,7 @@ def test_cluster_tables_match_oracle(engine, small, oracle):
         _report("cluster_vel_rel", ev)
         _report("cluster_cov_rel", es)
         assert ep < REL and ev < REL
-        assert es < COV_REL, i
+        assert es < REL, i
 
 
 def test_star_tracks_match_oracle(engine, small, oracle):
@@ -88,7 +88,7 @@ def test_star_tracks_match_oracle(engine, small, oracle):
         _report("star_pos_rel", ep)
         _report("star_cov_rel", es)
         assert ep < REL
-        assert es < COV_REL
+        assert es < REL
 
 
 def test_full_scan_chi2_matches_oracle(engine, small, oracle):
@@ -101,11 +101,12 @@ def test_full_scan_chi2_matches_oracle(engine, small, oracle):
     _report("chi2_rel_max", rel.max())
     _report("chi2_rel_p99", np.quantile(rel, 0.99))
     _report("chi2_rel_frac_above_1e-9", np.mean(rel > 1e-9))
-    assert rel.max() < COV_REL, rel.max()
-    # the minimum's position: the same point, except where two points' forms agree to rounding
-    same = (gkb == kb) & (np.abs(gsb - sb) <= 1e-9)
+    _report("chi2_identical_frac", np.mean(rel == 0))
+    assert rel.max() < REL, rel.max()
+    # the minimum's position: the same point
+    same = (gkb == kb) & (gsb == sb)
     _report("kb_sb_mismatch_frac", 1.0 - same.mean())
-    assert same.mean() > 0.999
+    assert same.all()
 
 
 def test_screen_rows_match_oracle(engine, small, oracle):
@@ -120,7 +121,7 @@ def test_screen_rows_match_oracle(engine, small, oracle):
         assert np.allclose(got[f], want[f], rtol=REL, atol=1e-12), f
     for f in ("chi2", "sig_pc"):
         _report(f"row_{f}_rel", np.max(np.abs(got[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-300)))
-        assert np.allclose(got[f], want[f], rtol=COV_REL, atol=1e-12), f
+        assert np.allclose(got[f], want[f], rtol=REL, atol=1e-12), f
 
 
 def test_screen_rows_text_equal_reference(engine, small, golden_dir):
@@ -204,7 +205,7 @@ def test_pg_abi_slices_match_oracle(small, oracle):
     chi2 = get(d_chi2, np.zeros(nb * nc))
     want, _, _ = oracle.chi2_matrix(cl, st)
     fin = np.isfinite(want.ravel())
-    assert np.max(np.abs(chi2[fin] - want.ravel()[fin]) / want.ravel()[fin]) < COV_REL
+    assert np.max(np.abs(chi2[fin] - want.ravel()[fin]) / want.ravel()[fin]) < REL
     # finish the pairs below the cut, in the reference driver's order
     kb = get(d_kb, np.zeros(nb * nc, np.int32))
     sb = get(d_sb, np.zeros(nb * nc))
@@ -233,7 +234,7 @@ def test_pg_abi_slices_match_oracle(small, oracle):
     res = get(d_res, np.zeros((m, 3)))
     b, t, v = oracle.pair_clones(ps, steps, dt)
     assert np.allclose(res[:, 0], b, rtol=REL) and np.allclose(res[:, 2], v, rtol=REL)
-    assert np.allclose(res[:, 1], t, rtol=1e-7, atol=1e-9)
+    assert np.allclose(res[:, 1], t, rtol=REL, atol=1e-12)
     for b_ in bufs:
         L.pg_free(b_)
 
@@ -253,8 +254,8 @@ def test_pair_plain_matches_oracle_and_reference_rows(small, oracle, golden_dir)
     for f in ("d05", "d50", "d95", "v05", "v50", "v95"):
         assert np.allclose(got[f], want[f], rtol=REL), f
     for f in ("t05", "t50", "t95"):
-        assert np.allclose(got[f], want[f], rtol=1e-7, atol=1e-9), f
+        assert np.allclose(got[f], want[f], rtol=REL, atol=1e-12), f
     with open(os.path.join(golden_dir, "rows_pair.txt")) as f_:
         ref = f_.read().splitlines()
     lines = prows.pair_lines(got, ti, ci, pul, cl)
-    assert sum(a != b for a, b in zip(lines, ref)) <= 1 and len(lines) == len(ref)
+    assert lines == ref

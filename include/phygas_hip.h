@@ -148,6 +148,19 @@ int phx_pair(PhxCtx *ctx, int form, const PhTarget *t, const PhCluster *c, int n
 enum { PHX_DRAW_ON_DEVICE = 1 };
 int phx_pair_ex(PhxCtx *ctx, int form, int flags, const PhTarget *t, const PhCluster *c, int npairs, const int *ti,
                 const int *ci, int n_clones, double dt_myr, PhResult *res);
+/* The Monte Carlo pair search for pairs of the loaded stars and the resident clusters (star = index in the
+ * loaded batch, in load order; cluster = index in the cluster table): e.g. stage 2 over the rows of a screening
+ * pass above a p_r50 cut, which the reference's GPU app runs over a pair list with its clones batched across
+ * pairs (reference: src/main_phygas_boinc.cpp:187-301).  Clones are drawn on the device from the resident
+ * records (the reference RNG stream of ph_seed(id, name)), traced in one persistent launch ordered longest
+ * trace-back first and reduced on the device; res in the caller's pair order.  Any clone count (pairs above
+ * 4,096 clones are reduced by radix selection).  With graphs on (phx_set_graphs) the batch's launches are
+ * captured once and replayed while its shape stays the same.  Pair forms: PHX_FORM_PLAIN (FP64) or
+ * PHX_FORM_FAST (FP32 force); PHX_FORM_FAST64 runs the FP64 clones. */
+int phx_pair_loaded(PhxCtx *ctx, int form, int npairs, const int *star, const int *cluster, int n_clones,
+                    double dt_myr, PhResult *res);
+/* Captures and replays of the device-draw Monte Carlo graphs. */
+int phx_pair_graph_stats(PhxCtx *ctx, long *captures, long *replays);
 
 /* ---------------------------------------------------------------- 3. serving loop
  * Requests (screening chunks, Monte Carlo pair jobs) submitted as they arrive and batched continuously onto
@@ -194,15 +207,20 @@ int phx_server_release(PhxServer *srv, long id);
  * "phygas_checkpoint 2 <targets_done> <lines_done>" is written atomically; a resumed run keeps exactly that many
  * complete lines of `out`, so no row is lost or duplicated. */
 typedef struct {
-    int form;             /* PHX_FORM_FAST (default), PHX_FORM_FAST64 or PHX_FORM_PLAIN */
+    int form;             /* screening: PHX_FORM_FAST64 (default: rows byte-identical to PHX_FORM_PLAIN's, the
+                             reference's results), PHX_FORM_PLAIN or PHX_FORM_FAST (FP32 tracks: opt-in) */
     int batch_targets;    /* targets per GPU batch (checkpoint interval); default 20,000 */
-    int max_batches;      /* > 0: stop (return 1) after this many batches, checkpoint written */
+    int max_batches;      /* > 0: stop (return PHX_WU_STOPPED) after this many batches, checkpoint written */
     int device;
     int draw_on_device;   /* pair mode: clones drawn on the GPU (PHX_DRAW_ON_DEVICE) instead of the host */
+    int pair_form;        /* pair mode: PHX_FORM_PLAIN (default: FP64 clones) or PHX_FORM_FAST (FP32 force) */
 } PhxWuOptions;
 void phx_wu_defaults(PhxWuOptions *opt);
-/* 0: done (out ends with "# done"); 1: stopped after max_batches; 4: the checkpoint exists but cannot be read
- * (nothing touched: retry later); other: error (phx_last_error()).  *targets_done: targets in out. */
+/* Return codes of phx_wu_run besides 0 (done: out ends with "# done"): PHX_WU_STOPPED after max_batches (the
+ * checkpoint is written); PHX_WU_CKPT_UNREADABLE: the checkpoint exists but cannot be read (nothing touched:
+ * retry later); any other nonzero code is an error (phx_last_error()), a device failure included.
+ * *targets_done: targets in out. */
+enum { PHX_WU_CKPT_UNREADABLE = 4, PHX_WU_STOPPED = 6 };
 int phx_wu_run(const char *in_path, const char *clusters_path, const char *out_path, const char *ckpt_path,
                const PhxWuOptions *opt, long *targets_done);
 /* The resume step alone (host only): reads the checkpoint, trims `out` to the lines it counts (or truncates it

@@ -79,17 +79,6 @@ int launch_rows(const long *d_pair_idx, const long *d_ridx, const long *d_rcount
 int launch_prefilter(const PhTarget *d_t, const uint64_t *d_tseed, const int *d_tn, int nt, const PhCluster *d_c,
                      const uint64_t *d_cseed, const int *d_cn, int nc, int npts, const Pot &P, double *d_tpath,
                      double *d_tspr, double *d_cpath, double *d_cspr, unsigned char *d_mask, hipStream_t st);
-// Clone draws on the device, one wavefront per pair, in the reference's sequential order; pair p writes its
-// m[p] clones at ps[off[p] ..].
-int launch_pair_draw(const PhTarget *d_t, const PhCluster *d_c, const int *d_ti, const int *d_ci,
-                     const uint64_t *d_seed, const long *d_off, int npairs, int n_clones, const Pot &P, double *d_ps,
-                     int *d_m, hipStream_t st);
-int launch_pair_steps(const long *d_off, const int *d_m, const int *d_psteps, const double *d_pdt, int npairs,
-                      int n_clones, int *d_steps, double *d_dts, hipStream_t st);
-// Quantiles and fractions of each pair's m clones (res[n][3]: best kpc, tbest Myr, vbest kpc/Myr).
-int launch_pair_reduce(const double *d_res, const long *d_off, const int *d_m, const double *d_r50_pc,
-                       const double *d_rtot_pc, int npairs, PhResult *d_out, hipStream_t st);
-
 // ---- device-wide selection (select.hip) ---------------------------------------------------------
 // Indices i in [0, n) with val[i] < thr (strictly; NaN never), in increasing order, into d_idx[0 .. cap);
 // the full count into *d_count (device; may exceed cap, then only cap indices are written).

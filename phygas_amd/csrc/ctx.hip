@@ -17,6 +17,7 @@
 
 #include "common.h"
 #include "fast.h"
+#include "pairs.h"
 
 using namespace phx;
 
@@ -86,9 +87,11 @@ struct PhxCtx {
     int nstage = 0;
     hipEvent_t ev[MAX_STAGES + 1] = {};
     const char *stage_name[MAX_STAGES] = {};
-    // pairs
-    DBuf pps, psteps, pdts, pst, pres, poff, pm, pr50, prt, pout;
-    DBuf ptg, pcl, pti, pci, pseed, ppst, ppdt;   // device-side draws
+    // Monte Carlo batches: one buffer block (pair_layout) and the graph of the last device-draw batch
+    DBuf pmem;
+    hipGraphExec_t pgexec = nullptr;
+    uint64_t pgkey = 0;
+    long pcaptures = 0, preplays = 0;
 };
 
 static FastWork &part_fw(PhxCtx *c, int i) { return i == 0 ? c->fw : c->pfw[i]; }
@@ -137,9 +140,8 @@ extern "C" void phx_destroy(PhxCtx *c) {
     (void)hipStreamSynchronize(c->st);
     DBuf *bufs[] = {&c->cl, &c->cseed, &c->cnom, &c->snom, &c->cps0, &c->csig, &c->csoa, &c->cn, &c->R2, &c->r50, &c->rt,
                     &c->sr, &c->sseed, &c->sprefix, &c->sgroup, &c->ps0, &c->sig, &c->trk, &c->chi2,
-                    &c->kb, &c->sb, &c->seltmp, &c->idx, &c->counts, &c->fout, &c->ridx, &c->rows,
-                    &c->pps, &c->psteps, &c->pdts, &c->pst, &c->pres, &c->poff, &c->pm, &c->pr50,
-                    &c->prt, &c->pout, &c->ptg, &c->pcl, &c->pti, &c->pci, &c->pseed, &c->ppst, &c->ppdt};
+                    &c->kb, &c->sb, &c->seltmp, &c->idx, &c->counts, &c->fout, &c->ridx, &c->rows, &c->pmem};
+    if (c->pgexec) (void)hipGraphExecDestroy(c->pgexec);
     for (DBuf *b : bufs) b->release();
     fast_release(c->fast, c->fw);
     for (int i = 1; i < MAX_SPLIT; i++) {
@@ -584,20 +586,23 @@ extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
         PHX_CK(hipMemcpyAsync(rows, c->rows.p, sizeof(PhxRow) * m, hipMemcpyDeviceToHost, c->st));
         PHX_CK(hipStreamSynchronize(c->st));
     } else if (m > 0) {
-        // each part's rows (its star indices are local to the part), then in (star, cluster) order
+        // every part's rows (its star indices are local to the part, its rows in append order), sorted into
+        // (star, cluster) order before the first m are returned
+        std::vector<PhxRow> all(nr);
         long o = 0;
-        for (int i = 0; i < c->nparts && o < m; i++) {
-            const long mi = std::min(c->part_rows[i], m - o);
+        for (int i = 0; i < c->nparts; i++) {
+            const long mi = c->part_rows[i];
             if (mi <= 0) continue;
-            PHX_CK(hipMemcpyAsync(rows + o, fast_rows_ptr(part_fw(c, i)), sizeof(PhxRow) * mi, hipMemcpyDeviceToHost,
-                                  c->st));
+            PHX_CK(hipMemcpyAsync(all.data() + o, fast_rows_ptr(part_fw(c, i)), sizeof(PhxRow) * mi,
+                                  hipMemcpyDeviceToHost, c->st));
             PHX_CK(hipStreamSynchronize(c->st));
-            for (long r = o; r < o + mi; r++) rows[r].star += c->part_base[i];
+            for (long r = o; r < o + mi; r++) all[r].star += c->part_base[i];
             o += mi;
         }
-        std::sort(rows, rows + m, [](const PhxRow &a, const PhxRow &b) {
+        std::sort(all.begin(), all.begin() + o, [](const PhxRow &a, const PhxRow &b) {
             return a.star != b.star ? a.star < b.star : a.cluster < b.cluster;
         });
+        memcpy(rows, all.data(), sizeof(PhxRow) * (size_t)m);
     }
     return 0;
 }
@@ -659,41 +664,6 @@ extern "C" int phx_pair_draw(const PhTarget *t, const PhCluster *cl, int n_clone
     return m;
 }
 
-static int trace_clones(PhxCtx *c, int form, int n, int smax) {
-    if (form == PHX_FORM_PLAIN)
-        return launch_pair_plain(c->pps.as<double>(), c->psteps.as<int>(), c->pdts.as<double>(), c->pst.as<double>(),
-                                 c->pres.as<double>(), c->P, 0, smax, n, c->st);
-    return fast_pair_clones(c->pps.as<double>(), c->psteps.as<int>(), c->pdts.as<double>(), c->pres.as<double>(), c->P,
-                            n, c->st);
-}
-
-extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int *steps, const double *dts, int n,
-                               double *best, double *tbest, double *vbest) {
-    PHX_REQUIRE(n >= 0, "negative clone count");
-    PHX_CK(hipSetDevice(c->device));
-    if (n == 0) return 0;
-    if (c->pps.ensure(96 * (size_t)n) || c->psteps.ensure(4 * (size_t)n) || c->pdts.ensure(8 * (size_t)n) ||
-        c->pst.ensure(168 * (size_t)n) || c->pres.ensure(24 * (size_t)n))
-        return 1;
-    int smax = 0;
-    for (int i = 0; i < n; i++) smax = std::max(smax, steps[i]);
-    PHX_CK(hipMemcpyAsync(c->pps.p, ps, 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->psteps.p, steps, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pdts.p, dts, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    stage_begin(c);
-    if (trace_clones(c, form, n, smax)) return 1;
-    stage_mark(c, "clones");
-    std::vector<double> r(3 * (size_t)n);
-    PHX_CK(hipMemcpyAsync(r.data(), c->pres.p, 24 * (size_t)n, hipMemcpyDeviceToHost, c->st));
-    PHX_CK(hipStreamSynchronize(c->st));
-    for (int i = 0; i < n; i++) {
-        best[i] = r[3 * i];
-        tbest[i] = r[3 * i + 1];
-        vbest[i] = r[3 * i + 2];
-    }
-    return 0;
-}
-
 // Pair-mode prefilter on the device (reference: src/phygas_cli.c:103-117, src/phygas.c:98-168): target paths
 // over the longest cluster trace-back, cluster paths over their own; mask[j * nc + i].
 extern "C" int phx_prefilter(PhxCtx *c, const PhTarget *t, int nt, const PhCluster *cl, int nc, unsigned char *mask) {
@@ -735,8 +705,84 @@ extern "C" int phx_prefilter(PhxCtx *c, const PhTarget *t, int nt, const PhClust
     return rc;
 }
 
-// Clone draws on the device (launch_pair_draw): pair p owns n_clones slots at rank(p) * n_clones, pairs
-// ranked longest trace-back first; unused slots run 0 steps.
+// Device buffers of the context's Monte Carlo batches: one block, carved by pair_layout.
+static int pair_buffers(PhxCtx *c, long slots, int npairs, int ntg, int ncl, PairLayout &L) {
+    const size_t need = pair_layout(nullptr, slots, npairs, ntg, ncl).total;
+    if (c->pmem.ensure(need)) return 1;
+    L = pair_layout((char *)c->pmem.p, slots, npairs, ntg, ncl);
+    return 0;
+}
+
+static int check_pair_args(int &form, int npairs, int n_clones) {
+    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
+    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "unknown form");
+    PHX_REQUIRE((double)npairs * n_clones < 4.0e9, "too many clones in one batch (at most 4e9 slots)");
+    if (form == PHX_FORM_FAST64) form = PHX_FORM_PLAIN;   // FAST64's clones are the plain FP64 ones
+    return 0;
+}
+
+extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int *steps, const double *dts, int n,
+                               double *best, double *tbest, double *vbest) {
+    if (check_pair_args(form, 0, 0)) return 2;
+    PHX_REQUIRE(n >= 0, "negative clone count");
+    PHX_CK(hipSetDevice(c->device));
+    if (n == 0) return 0;
+    PairLayout L;
+    if (pair_buffers(c, n, 0, 0, 0, L)) return 1;
+    PHX_CK(hipMemcpyAsync(L.ps, ps, 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.steps, steps, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.dts, dts, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    stage_begin(c);
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, n, L.next, c->st)) return 1;
+    stage_mark(c, "clones");
+    std::vector<double> r(3 * (size_t)n);
+    PHX_CK(hipMemcpyAsync(r.data(), L.res, 24 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; i++) {
+        best[i] = r[3 * i];
+        tbest[i] = r[3 * i + 1];
+        vbest[i] = r[3 * i + 2];
+    }
+    return 0;
+}
+
+// The launches of a planned device-draw batch, captured into a hipGraph on first use and replayed while the
+// batch's sizes, form and buffers stay the same (when graphs are on).
+static int pair_run_launches(PhxCtx *c, const PairLayout &L, const PairPlan &plan, int form, const PhTarget *d_t,
+                             const PhCluster *d_c) {
+    stage_begin(c);
+    if (!c->fw.use_graphs || c->timing) {
+        if (pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st)) return 1;
+        stage_mark(c, "pairs");
+        return 0;
+    }
+    uint64_t key = FNV_BASIS;
+    auto mix = [&](uint64_t v) { key = (key ^ v) * FNV_PRIME; };
+    mix((uint64_t)plan.npairs); mix((uint64_t)plan.n_clones); mix((uint64_t)plan.slots); mix((uint64_t)form);
+    mix((uint64_t)(uintptr_t)L.ps); mix((uint64_t)(uintptr_t)d_t); mix((uint64_t)(uintptr_t)d_c);
+    if (!c->pgexec || c->pgkey != key) {
+        if (c->pgexec) (void)hipGraphExecDestroy(c->pgexec);
+        c->pgexec = nullptr;
+        hipGraph_t g = nullptr;
+        PHX_CK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        const int rc = pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st);
+        const hipError_t ec = hipStreamEndCapture(c->st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        PHX_CK(ec);
+        const hipError_t ei = hipGraphInstantiate(&c->pgexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PHX_CK(ei);
+        c->pgkey = key;
+        c->pcaptures++;
+    } else {
+        c->preplays++;
+    }
+    PHX_CK(hipGraphLaunch(c->pgexec, c->st));
+    stage_mark(c, "pairs");
+    return 0;
+}
+
+// Clone draws on the device from host-given records (copied into the batch's buffers).
 static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
                              const int *ci, int n_clones, double dt_myr, PhResult *res) {
     int nt = 0, ncl = 0;
@@ -744,131 +790,90 @@ static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhClu
         nt = std::max(nt, ti[p] + 1);
         ncl = std::max(ncl, ci[p] + 1);
     }
-    std::vector<uint64_t> seeds(npairs);
-    std::vector<int> pst(npairs), order(npairs);
-    std::vector<double> pdt(npairs), r50(npairs), rt(npairs);
-    for (int p = 0; p < npairs; p++) {
-        const PhTarget &T = t[ti[p]];
-        const PhCluster &C = cl[ci[p]];
-        seeds[p] = phx_seed(T.id, C.name);
-        pst[p] = (int)ceil(C.tmax_myr / dt_myr);   // reference: src/phygas.c:185-187
-        pdt[p] = -C.tmax_myr / pst[p];
-        r50[p] = C.r50_pc;
-        rt[p] = C.rtot_pc;
-        order[p] = p;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pst[a] > pst[b]; });
-    std::vector<long> off(npairs);
-    for (int r = 0; r < npairs; r++) off[order[r]] = (long)r * n_clones;
-    const size_t n = (size_t)npairs * n_clones;
-    if (c->pps.ensure(96 * n) || c->psteps.ensure(4 * n) || c->pdts.ensure(8 * n) || c->pst.ensure(168 * n) ||
-        c->pres.ensure(24 * n) || c->poff.ensure(8 * (size_t)npairs) || c->pm.ensure(4 * (size_t)npairs) ||
-        c->pr50.ensure(8 * (size_t)npairs) || c->prt.ensure(8 * (size_t)npairs) ||
-        c->pout.ensure(sizeof(PhResult) * (size_t)npairs) || c->ptg.ensure(sizeof(PhTarget) * nt) ||
-        c->pcl.ensure(sizeof(PhCluster) * ncl) || c->pti.ensure(4 * (size_t)npairs) || c->pci.ensure(4 * (size_t)npairs) ||
-        c->pseed.ensure(8 * (size_t)npairs) || c->ppst.ensure(4 * (size_t)npairs) || c->ppdt.ensure(8 * (size_t)npairs))
-        return 1;
-    PHX_CK(hipMemcpyAsync(c->ptg.p, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pcl.p, cl, sizeof(PhCluster) * ncl, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pti.p, ti, 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pci.p, ci, 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pseed.p, seeds.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->ppst.p, pst.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->ppdt.p, pdt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->poff.p, off.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pr50.p, r50.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->prt.p, rt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemsetAsync(c->pps.p, 0, 96 * n, c->st));
-    stage_begin(c);
-    if (launch_pair_draw(c->ptg.as<PhTarget>(), c->pcl.as<PhCluster>(), c->pti.as<int>(), c->pci.as<int>(),
-                         c->pseed.as<uint64_t>(), c->poff.as<long>(), npairs, n_clones, c->P, c->pps.as<double>(),
-                         c->pm.as<int>(), c->st) ||
-        launch_pair_steps(c->poff.as<long>(), c->pm.as<int>(), c->ppst.as<int>(), c->ppdt.as<double>(), npairs,
-                          n_clones, c->psteps.as<int>(), c->pdts.as<double>(), c->st))
-        return 1;
-    stage_mark(c, "draws");
-    const int smax = *std::max_element(pst.begin(), pst.end());
-    if (trace_clones(c, form, (int)n, smax)) return 1;
-    stage_mark(c, "clones");
-    if (launch_pair_reduce(c->pres.as<double>(), c->poff.as<long>(), c->pm.as<int>(), c->pr50.as<double>(),
-                           c->prt.as<double>(), npairs, c->pout.as<PhResult>(), c->st))
-        return 1;
-    stage_mark(c, "reduce");
-    PHX_CK(hipMemcpyAsync(res, c->pout.p, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
+    PairPlan plan;
+    pair_plan(t, cl, npairs, ti, ci, n_clones, dt_myr, plan);
+    PairLayout L;
+    if (pair_buffers(c, plan.slots, npairs, nt, ncl, L)) return 1;
+    PHX_CK(hipMemcpyAsync(L.tg, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.cl, cl, sizeof(PhCluster) * ncl, hipMemcpyHostToDevice, c->st));
+    if (pair_upload_plan(L, plan, ti, ci, c->st)) return 1;
+    if (pair_run_launches(c, L, plan, form, L.tg, L.cl)) return 1;
+    PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
     PHX_CK(hipStreamSynchronize(c->st));
     return 0;
 }
 
 extern "C" int phx_pair_ex(PhxCtx *c, int form, int flags, const PhTarget *t, const PhCluster *cl, int npairs,
                            const int *ti, const int *ci, int n_clones, double dt_myr, PhResult *res) {
-    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
-    PHX_REQUIRE(n_clones <= 4096, "at most 4096 clones per pair (device reduction size)");
-    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST, "unknown form");
+    if (check_pair_args(form, npairs, n_clones)) return 2;
     PHX_CK(hipSetDevice(c->device));
     if (npairs == 0) return 0;
     if (flags & PHX_DRAW_ON_DEVICE) return pair_device_draws(c, form, t, cl, npairs, ti, ci, n_clones, dt_myr, res);
     return phx_pair(c, form, t, cl, npairs, ti, ci, n_clones, dt_myr, res);
 }
 
-extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
-                        const int *ci, int n_clones, double dt_myr, PhResult *res) {
-    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
-    PHX_REQUIRE(n_clones <= 4096, "at most 4096 clones per pair (device reduction size)");
-    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST, "unknown form");
+extern "C" int phx_pair_loaded(PhxCtx *c, int form, int npairs, const int *star, const int *cluster, int n_clones,
+                               double dt_myr, PhResult *res) {
+    if (check_pair_args(form, npairs, n_clones)) return 2;
+    PHX_REQUIRE(c->nc > 0, "load clusters first");
     PHX_CK(hipSetDevice(c->device));
     if (npairs == 0) return 0;
-    // draws per pair (host, reference RNG), then the device layout: pairs longest trace-back first so
-    // every wavefront's clones run the same number of steps; results stay in the caller's pair order
-    std::vector<double> draw((size_t)npairs * n_clones * 12);
-    std::vector<int> m(npairs), pst(npairs), order(npairs);
-    std::vector<double> pdt(npairs), r50(npairs), rt(npairs);
+    const int ns = (int)c->st_h.size();
+    for (int p = 0; p < npairs; p++)
+        PHX_REQUIRE(star[p] >= 0 && star[p] < ns && cluster[p] >= 0 && cluster[p] < c->nc, "pair index out of range");
+    if (upload_stars(c)) return 1;
+    PairPlan plan;
+    pair_plan(c->st_h.data(), c->cl_h.data(), npairs, star, cluster, n_clones, dt_myr, plan);
+    PairLayout L;
+    if (pair_buffers(c, plan.slots, npairs, 0, 0, L)) return 1;
+    if (pair_upload_plan(L, plan, star, cluster, c->st)) return 1;
+    if (pair_run_launches(c, L, plan, form, c->sr.as<PhTarget>(), c->cl.as<PhCluster>())) return 1;
+    PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_pair_graph_stats(PhxCtx *c, long *captures, long *replays) {
+    *captures = c->pcaptures;
+    *replays = c->preplays;
+    return 0;
+}
+
+extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster *cl, int npairs, const int *ti,
+                        const int *ci, int n_clones, double dt_myr, PhResult *res) {
+    if (check_pair_args(form, npairs, n_clones)) return 2;
+    PHX_CK(hipSetDevice(c->device));
+    if (npairs == 0) return 0;
+    // draws per pair on the host (reference RNG), into the slots of the device layout: pairs longest trace-back
+    // first, so the clone queue hands out the long chunks first; results stay in the caller's pair order
+    PairPlan plan;
+    pair_plan(t, cl, npairs, ti, ci, n_clones, dt_myr, plan);
+    const size_t n = (size_t)plan.slots;
+    std::vector<double> ps(n * 12, 0.0), dts(n, 0.0);
+    std::vector<int> steps(n, 0), m(npairs);
     for (int p = 0; p < npairs; p++) {
-        const PhTarget &T = t[ti[p]];
-        const PhCluster &C = cl[ci[p]];
-        m[p] = phx_pair_draw(&T, &C, n_clones, dt_myr, phx_seed(T.id, C.name), draw.data() + (size_t)p * n_clones * 12,
-                             &pst[p], &pdt[p]);
-        r50[p] = C.r50_pc;
-        rt[p] = C.rtot_pc;
-        order[p] = p;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pst[a] > pst[b]; });
-    std::vector<double> ps((size_t)npairs * n_clones * 12);
-    std::vector<int> steps((size_t)npairs * n_clones);
-    std::vector<double> dts((size_t)npairs * n_clones);
-    std::vector<long> off(npairs);
-    long n = 0;
-    int smax = 0;
-    for (int p : order) {
-        off[p] = n;
-        memcpy(ps.data() + (size_t)n * 12, draw.data() + (size_t)p * n_clones * 12, sizeof(double) * 12 * m[p]);
-        for (int k = 0; k < m[p]; k++) {
-            steps[n + k] = pst[p];
-            dts[n + k] = pdt[p];
+        int st_p = 0;
+        double dt_p = 0;
+        m[p] = phx_pair_draw(&t[ti[p]], &cl[ci[p]], n_clones, dt_myr, plan.seed[p], ps.data() + (size_t)plan.off[p] * 12,
+                             &st_p, &dt_p);
+        for (int k = 0; k < n_clones; k++) {
+            steps[plan.off[p] + k] = k < m[p] ? st_p : 0;
+            dts[plan.off[p] + k] = dt_p;
         }
-        smax = std::max(smax, pst[p]);
-        n += m[p];
     }
-    if (c->pps.ensure(96 * (size_t)std::max(n, 1L)) || c->psteps.ensure(4 * (size_t)std::max(n, 1L)) ||
-        c->pdts.ensure(8 * (size_t)std::max(n, 1L)) || c->pst.ensure(168 * (size_t)std::max(n, 1L)) ||
-        c->pres.ensure(24 * (size_t)std::max(n, 1L)) || c->poff.ensure(8 * (size_t)npairs) ||
-        c->pm.ensure(4 * (size_t)npairs) || c->pr50.ensure(8 * (size_t)npairs) || c->prt.ensure(8 * (size_t)npairs) ||
-        c->pout.ensure(sizeof(PhResult) * (size_t)npairs))
-        return 1;
-    PHX_CK(hipMemcpyAsync(c->pps.p, ps.data(), 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->psteps.p, steps.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pdts.p, dts.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->poff.p, off.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pm.p, m.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->pr50.p, r50.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
-    PHX_CK(hipMemcpyAsync(c->prt.p, rt.data(), 8 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
+    PairLayout L;
+    if (pair_buffers(c, plan.slots, npairs, 0, 0, L)) return 1;
+    if (pair_upload_plan(L, plan, ti, ci, c->st)) return 1;
+    PHX_CK(hipMemcpyAsync(L.ps, ps.data(), 96 * n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.steps, steps.data(), 4 * n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.dts, dts.data(), 8 * n, hipMemcpyHostToDevice, c->st));
+    PHX_CK(hipMemcpyAsync(L.m, m.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
     stage_begin(c);
-    if (n > 0 && trace_clones(c, form, (int)n, smax)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, plan.slots, L.next, c->st)) return 1;
     stage_mark(c, "clones");
-    if (launch_pair_reduce(c->pres.as<double>(), c->poff.as<long>(), c->pm.as<int>(), c->pr50.as<double>(),
-                           c->prt.as<double>(), npairs, c->pout.as<PhResult>(), c->st))
-        return 1;
+    if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, npairs, n_clones, L.out, c->st)) return 1;
     stage_mark(c, "reduce");
-    PHX_CK(hipMemcpyAsync(res, c->pout.p, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
+    PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
     PHX_CK(hipStreamSynchronize(c->st));
     return 0;
 }

@@ -65,7 +65,6 @@ const double *fast_tracks_ptr(const FastWork &W);
 int fast_z0_check(uint64_t seed, long n, double *worst, hipStream_t st);
 void fast_release(FastTables &T, FastWork &W);
 void fast_release_work(FastWork &W);   // the work buffers and graph alone
-int fast_pair_clones(const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P, int n,
-                     hipStream_t st);
+
 
 }  // namespace phx

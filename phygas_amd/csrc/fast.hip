@@ -1524,53 +1524,4 @@ void fast_release(FastTables &T, FastWork &W) {
     fast_release_work(W);
 }
 
-// ------------------------------------------------------------------------------------------- clones
-// One lane per clone (reference: src/phygas.c:200-216): both bodies kick-drift-kick with the FP32 force
-// evaluated at the FP64 position, FP64 position / velocity accumulators, and the closest approach of each
-// step's segment refined in FP64 with the reference's strict-< first-step rule.  Callers lay clones out by
-// pair, longest trace-back first, so a wavefront's lanes run the same number of steps.
-__global__ __launch_bounds__(256) void k_pair_fast(const double *__restrict__ ps, const int *__restrict__ steps,
-                                                   const double *__restrict__ dts, double *__restrict__ res, PotF Q,
-                                                   int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double *p = ps + (size_t)k * 12;
-    V3 ap = v3(p[0], p[1], p[2]), av = v3(p[3], p[4], p[5]), bp = v3(p[6], p[7], p[8]), bv = v3(p[9], p[10], p[11]);
-    F3 fa = force32(Q, (float)ap.x, (float)ap.y, (float)ap.z), fb = force32(Q, (float)bp.x, (float)bp.y, (float)bp.z);
-    V3 r0 = ap - bp;
-    double best = norm(r0), tbest = 0.0, vbest = norm(av - bv);
-    const int ns = steps[k];
-    const double dt = dts[k], h = 0.5 * dt;
-    for (int i = 1; i <= ns; i++) {
-        av = av + v3(fa.x, fa.y, fa.z) * h;
-        bv = bv + v3(fb.x, fb.y, fb.z) * h;
-        ap = ap + av * dt;
-        bp = bp + bv * dt;
-        fa = force32(Q, (float)ap.x, (float)ap.y, (float)ap.z);
-        fb = force32(Q, (float)bp.x, (float)bp.y, (float)bp.z);
-        av = av + v3(fa.x, fa.y, fa.z) * h;
-        bv = bv + v3(fb.x, fb.y, fb.z) * h;
-        const V3 r1 = ap - bp;
-        const double s = seg_param(r0, r1);
-        const double dm = norm(r0 + (r1 - r0) * s);
-        if (dm < best) {
-            best = dm;
-            tbest = (i - 1 + s) * -dt;
-            vbest = norm(av - bv);
-        }
-        r0 = r1;
-    }
-    double *R = res + (size_t)k * 3;
-    R[0] = best; R[1] = tbest; R[2] = vbest;
-}
-
-int fast_pair_clones(const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P, int n,
-                     hipStream_t st) {
-    if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_pair_fast, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res,
-                       make_potf(P), n);
-    PHX_CK(hipGetLastError());
-    return 0;
-}
-
 }  // namespace phx

@@ -454,172 +454,6 @@ int launch_prefilter(const PhTarget *d_t, const uint64_t *d_tseed, const int *d_
     return 0;
 }
 
-// ---------------------------------------------------------------- Monte Carlo draws on the device
-// The clones of a pair are drawn from one splitmix64 stream in clone order (reference: src/phygas.c:182-198):
-// target draw then cluster draw, each retried while the parallax / distance is not positive, a clone that
-// fails either draw skipped.  splitmix64's state after i draws is seed + i*gamma, so one wavefront per pair
-// draws 64 consecutive attempts at once, each assuming that every attempt before it used the regular
-// 16 uniforms (8 per body: parallax or distance, pmra, pmdec, RV).  Attempts up to and including the first
-// one that used a different count are exact; the next round starts after that attempt's true end state.
-// Output order and values are those of the sequential loop.
-constexpr uint64_t GAMMA_INV = 0xF1DE83E19937733DULL;   // gamma^-1 mod 2^64
-
-__global__ __launch_bounds__(64) void k_pair_draw(const PhTarget *__restrict__ tg, const PhCluster *__restrict__ cl,
-                                                 const int *__restrict__ ti, const int *__restrict__ ci,
-                                                 const uint64_t *__restrict__ seeds, const long *__restrict__ off,
-                                                 int n_clones, Pot P, double *__restrict__ ps, int *__restrict__ m_out) {
-    const int p = blockIdx.x, lane = threadIdx.x;
-    const PhTarget T = tg[ti[p]];
-    const PhCluster Cc = cl[ci[p]];
-    double *out = ps + (size_t)off[p] * 12;
-    uint64_t base = seeds[p];
-    int attempt = 0, made = 0;
-    while (attempt < n_clones) {
-        const int k = attempt + lane;
-        uint64_t st = base + (uint64_t)(16 * lane) * GOLDEN_GAMMA;
-        const uint64_t st0 = st;
-        V3 ap, av, bp, bv;
-        int ok = 0;
-        if (k < n_clones) ok = (sample_target(P, T, &st, ap, av) == 0 && sample_cluster(P, Cc, &st, bp, bv) == 0);
-        const uint64_t used = (st - st0) * GAMMA_INV;
-        const bool irregular = k < n_clones && used != 16;
-        const unsigned long long irr = __ballot(irregular);
-        const int last = irr ? __builtin_ctzll(irr) : 63;          // lanes 0..last are exact
-        const unsigned long long oks = __ballot(ok && lane <= last);
-        if (ok && lane <= last) {
-            double *o = out + (size_t)(made + __popcll(oks & ((1ull << lane) - 1ull))) * 12;
-            o[0] = ap.x; o[1] = ap.y; o[2] = ap.z; o[3] = av.x; o[4] = av.y; o[5] = av.z;
-            o[6] = bp.x; o[7] = bp.y; o[8] = bp.z; o[9] = bv.x; o[10] = bv.y; o[11] = bv.z;
-        }
-        made += __popcll(oks);
-        // next round starts after lane `last`'s true end state
-        const uint64_t end_last = __shfl(st, last, 64);
-        base = irr ? end_last : base + (uint64_t)(16 * 64) * GOLDEN_GAMMA;
-        attempt += last + 1;
-    }
-    if (lane == 0) m_out[p] = made;
-}
-
-int launch_pair_draw(const PhTarget *d_t, const PhCluster *d_c, const int *d_ti, const int *d_ci,
-                     const uint64_t *d_seed, const long *d_off, int npairs, int n_clones, const Pot &P, double *d_ps,
-                     int *d_m, hipStream_t st) {
-    if (npairs <= 0) return 0;
-    hipLaunchKernelGGL(k_pair_draw, dim3(npairs), dim3(64), 0, st, d_t, d_c, d_ti, d_ci, d_seed, d_off, n_clones, P,
-                       d_ps, d_m);
-    PHX_CK(hipGetLastError());
-    return 0;
-}
-
-// Steps and dt of every clone slot from its pair: pair p owns slots [off[p], off[p] + n_clones), the first
-// m[p] hold clones, the rest get 0 steps.
-__global__ __launch_bounds__(256) void k_pair_steps(const long *__restrict__ off, const int *__restrict__ m,
-                                                    const int *__restrict__ psteps, const double *__restrict__ pdt,
-                                                    int n_clones, int *__restrict__ steps, double *__restrict__ dts) {
-    const int p = blockIdx.x;
-    for (int k = threadIdx.x; k < n_clones; k += blockDim.x) {
-        steps[off[p] + k] = k < m[p] ? psteps[p] : 0;
-        dts[off[p] + k] = pdt[p];
-    }
-}
-
-int launch_pair_steps(const long *d_off, const int *d_m, const int *d_psteps, const double *d_pdt, int npairs,
-                      int n_clones, int *d_steps, double *d_dts, hipStream_t st) {
-    if (npairs <= 0) return 0;
-    hipLaunchKernelGGL(k_pair_steps, dim3(npairs), dim3(256), 0, st, d_off, d_m, d_psteps, d_pdt, n_clones, d_steps,
-                       d_dts);
-    PHX_CK(hipGetLastError());
-    return 0;
-}
-
-// ---------------------------------------------------------------- Monte Carlo reduction
-// One workgroup per pair: the m clone results are converted to pc / Myr / km/s, counted against r50 and
-// rtot, and each column is sorted in LDS (bitonic, padded with +inf to a power of two) so the 5/50/95 %
-// quantiles interpolate exactly as the reference's qsort + quantile (reference: src/phygas.c:170-180,
-// 218-238).
-constexpr int RED_MAX = 4096;
-
-__device__ void bitonic_sort_lds(double *v, int n2) {
-    for (int k = 2; k <= n2; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-                int l = i ^ j;
-                if (l > i) {
-                    bool up = (i & k) == 0;
-                    double a = v[i], b = v[l];
-                    if ((a > b) == up) { v[i] = b; v[l] = a; }
-                }
-            }
-            __syncthreads();
-        }
-}
-
-__device__ double quantile_sorted(const double *v, int n, double q) {
-    double pos = q * (n - 1);
-    int i = (int)pos;
-    if (i >= n - 1) return v[n - 1];
-    return v[i] + (pos - i) * (v[i + 1] - v[i]);
-}
-
-__global__ __launch_bounds__(256) void k_pair_reduce(const double *__restrict__ res, const long *__restrict__ off,
-                                                     const int *__restrict__ m_arr, const double *__restrict__ r50_pc,
-                                                     const double *__restrict__ rtot_pc, PhResult *__restrict__ out) {
-    __shared__ double v[RED_MAX];
-    __shared__ int cnt[2];
-    const int p = blockIdx.x, m = m_arr[p];
-    const double *R = res + (size_t)off[p] * 3;
-    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    int a = 0, b = 0;
-    for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        double d = R[k * 3] * 1000.0;
-        a += d < r50_pc[p];
-        b += d < rtot_pc[p];
-    }
-    atomicAdd(&cnt[0], a);
-    atomicAdd(&cnt[1], b);
-    int n2 = 1;
-    while (n2 < m) n2 <<= 1;
-    PhResult o;
-    memset(&o, 0, sizeof o);
-    o.n = m;
-    for (int col = 0; col < 3; col++) {
-        __syncthreads();
-        for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-            double x = __builtin_inf();
-            if (k < m) {
-                x = R[k * 3 + col];
-                if (col == 0) x = x * 1000.0;
-                else if (col == 2) x = x * KMS_PER_KPCMYR;
-            }
-            v[k] = x;
-        }
-        __syncthreads();
-        bitonic_sort_lds(v, n2);
-        if (threadIdx.x == 0 && m > 0) {
-            double q05 = quantile_sorted(v, m, 0.05), q50 = quantile_sorted(v, m, 0.5), q95 = quantile_sorted(v, m, 0.95);
-            if (col == 0) { o.d05 = q05; o.d50 = q50; o.d95 = q95; }
-            else if (col == 1) { o.t05 = q05; o.t50 = q50; o.t95 = q95; }
-            else { o.v05 = q05; o.v50 = q50; o.v95 = q95; }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (m > 0) {
-            o.f_r50 = (double)cnt[0] / m;
-            o.f_rtot = (double)cnt[1] / m;
-        }
-        out[p] = o;
-    }
-}
-
-int launch_pair_reduce(const double *d_res, const long *d_off, const int *d_m, const double *d_r50_pc,
-                       const double *d_rtot_pc, int npairs, PhResult *d_out, hipStream_t st) {
-    if (npairs <= 0) return 0;
-    hipLaunchKernelGGL(k_pair_reduce, dim3(npairs), dim3(256), 0, st, d_res, d_off, d_m, d_r50_pc, d_rtot_pc, d_out);
-    PHX_CK(hipGetLastError());
-    return 0;
-}
-
 // ---------------------------------------------------------------- Monte Carlo clones
 // One lane per clone: target and cluster orbits stepped together, closest approach on each step's
 // segment, strict-< minimum (reference: src/phygas.c:200-216; device src/phygas_kernel_body.h:225-274).

@@ -179,7 +179,7 @@ extern "C" int phx_wu_resume(const char *out_path, const char *ckpt_path, long n
         FILE *f = fopen(ckpt_path, "rb");
         if (!f) {
             set_error("phx_wu_resume", "the checkpoint exists but cannot be read: retry later (out left alone)");
-            return 4;
+            return PHX_WU_CKPT_UNREADABLE;
         }
         char buf[256] = {0};
         const size_t n = fread(buf, 1, sizeof buf - 1, f);
@@ -187,7 +187,7 @@ extern "C" int phx_wu_resume(const char *out_path, const char *ckpt_path, long n
         fclose(f);
         if (err) {
             set_error("phx_wu_resume", "the checkpoint exists but cannot be read: retry later (out left alone)");
-            return 4;
+            return PHX_WU_CKPT_UNREADABLE;
         }
         buf[n] = 0;
         char tag[32] = {0};
@@ -220,13 +220,18 @@ extern "C" int phx_wu_resume(const char *out_path, const char *ckpt_path, long n
     return 0;
 }
 
+// Defaults give the reference's results: screening rows byte-identical to the plain FP64 form (FAST64: the
+// plain tracks, the exact culled scan and the screened finish), pair rows from FP64 clones.  The FP32 forms
+// change results in the last printed digits, so they are opt-in, as the reference keeps result-changing
+// variants behind a switch (its replicas are compared by a validator).
 extern "C" void phx_wu_defaults(PhxWuOptions *o) {
     memset(o, 0, sizeof *o);
-    o->form = PHX_FORM_FAST;
+    o->form = PHX_FORM_FAST64;
     o->batch_targets = 20000;
     o->max_batches = 0;
     o->device = 0;
     o->draw_on_device = 0;
+    o->pair_form = PHX_FORM_PLAIN;
 }
 
 namespace {
@@ -313,7 +318,7 @@ extern "C" int phx_wu_run(const char *in_path, const char *clusters_path, const 
         rc = phx_load_clusters(ctx, cl.data(), nc, n_samp, p[2] > 0 ? p[2] : 2.0);
         std::vector<PhxRow> rows;
         for (long b0 = start; rc == 0 && b0 < nt; b0 += B) {
-            if (o.max_batches > 0 && batches == o.max_batches) { rc = 1; break; }
+            if (o.max_batches > 0 && batches == o.max_batches) { rc = PHX_WU_STOPPED; break; }
             const int nb = (int)std::min(B, nt - b0);
             long n = 0;
             if ((rc = phx_clear_stars(ctx)) || (rc = phx_load_stars(ctx, tg.data() + b0, nb, 0)) ||
@@ -339,7 +344,7 @@ extern "C" int phx_wu_run(const char *in_path, const char *clusters_path, const 
         std::vector<int> ti, ci;
         std::vector<PhResult> res;
         for (long b0 = start; rc == 0 && b0 < nt; b0 += B) {
-            if (o.max_batches > 0 && batches == o.max_batches) { rc = 1; break; }
+            if (o.max_batches > 0 && batches == o.max_batches) { rc = PHX_WU_STOPPED; break; }
             const int nb = (int)std::min(B, nt - b0);
             mask.assign((size_t)nb * nc, 0);
             if ((rc = phx_prefilter(ctx, tg.data() + b0, nb, cl.data(), nc, mask.data()))) break;
@@ -354,7 +359,7 @@ extern "C" int phx_wu_run(const char *in_path, const char *clusters_path, const 
                     }
                 }
             res.resize(ti.size());
-            if (!ti.empty() && (rc = phx_pair_ex(ctx, o.form, o.draw_on_device ? PHX_DRAW_ON_DEVICE : 0, tg.data(),
+            if (!ti.empty() && (rc = phx_pair_ex(ctx, o.pair_form, o.draw_on_device ? PHX_DRAW_ON_DEVICE : 0, tg.data(),
                                                  cl.data(), (int)ti.size(), ti.data(), ci.data(), n_clones, dt_myr,
                                                  res.data())))
                 break;

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         "phx_pass_stats": (_I, [_P, _P, _I]),
         "phx_wu_run": (_I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _P, C.POINTER(_L)]),
         "phx_wu_resume": (_I, [C.c_char_p, C.c_char_p, _L, C.POINTER(_L), C.POINTER(_L)]),
+        "phx_wu_defaults": (None, [_P]),
         "phx_stage_times": (_I, [_P, _P, _P, _I]), "phx_set_timing": (None, [_P, _I]),
         "phx_cluster_tracks": (_I, [_P, _P, _P, _P, _P, _I]), "phx_star_tracks": (_I, [_P, _I, _I, _P, _I]),
         "phx_chi2_matrix": (_I, [_P, _P, _P, _P]), "phx_npts": (_I, [_P]),
@@ -75,6 +76,8 @@ def lib() -> C.CDLL:
         "phx_check_finish_screen": (_I, [_P, C.c_uint64, _L, _P]),
         "phx_check_crmath": (_I, [_I, _I, _P, _P, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
+        "phx_pair_loaded": (_I, [_P, _I, _I, _P, _P, _I, _D, _P]),
+        "phx_pair_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
         "phx_set_grids": (_I, [_P, _P]),
@@ -133,19 +136,38 @@ def crmath(fn: str, x, y=None, device: bool = False) -> np.ndarray:
 class WuOptions(C.Structure):
     """PhxWuOptions of include/phygas_hip.h."""
     _fields_ = [("form", C.c_int), ("batch_targets", C.c_int), ("max_batches", C.c_int), ("device", C.c_int),
-                ("draw_on_device", C.c_int)]
+                ("draw_on_device", C.c_int), ("pair_form", C.c_int)]
 
 
-def wu_run(in_path: str, clusters_path: str, out_path: str, ckpt_path: str, form: int = FAST,
-           batch_targets: int = 20000, max_batches: int = 0, device: int = 0, draw_on_device: bool = False) -> tuple[int, int]:
-    """Run a work unit (phx_wu_run): returns (status, targets done); status 0 = done, 1 = stopped after
-    max_batches, 4 = the checkpoint exists but cannot be read.  Other codes raise."""
+WU_DONE, WU_CKPT_UNREADABLE, WU_STOPPED = 0, 4, 6
+
+
+def wu_defaults() -> WuOptions:
+    """The library's default work-unit options (phx_wu_defaults)."""
+    o = WuOptions()
+    lib().phx_wu_defaults(C.byref(o))
+    return o
+
+
+def wu_run(in_path: str, clusters_path: str, out_path: str, ckpt_path: str, form: int | None = None,
+           batch_targets: int | None = None, max_batches: int = 0, device: int = 0, draw_on_device: bool = False,
+           pair_form: int | None = None) -> tuple[int, int]:
+    """Run a work unit (phx_wu_run): returns (status, targets done); status WU_DONE, WU_STOPPED (after
+    max_batches) or WU_CKPT_UNREADABLE (the checkpoint exists but cannot be read).  Any other code is an error
+    and raises.  Options left at None take the library's defaults (phx_wu_defaults)."""
     L = lib()
-    o = WuOptions(form, batch_targets, max_batches, device, int(draw_on_device))
+    o = wu_defaults()
+    if form is not None:
+        o.form = form
+    if pair_form is not None:
+        o.pair_form = pair_form
+    if batch_targets is not None:
+        o.batch_targets = batch_targets
+    o.max_batches, o.device, o.draw_on_device = max_batches, device, int(draw_on_device)
     done = _L(0)
     rc = L.phx_wu_run(in_path.encode(), clusters_path.encode(), out_path.encode(), ckpt_path.encode(),
                       C.addressof(o), C.byref(done))
-    if rc not in (0, 1, 4):
+    if rc not in (WU_DONE, WU_STOPPED, WU_CKPT_UNREADABLE):
         _check(rc, "phx_wu_run")
     return rc, done.value
 
@@ -213,12 +235,14 @@ class Engine:
     def screen_async(self, form: int = PLAIN, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0):
         _check(self._L.phx_screen(self.h, form, chi2_max, n_samp, p_min), "phx_screen")
 
-    def rows(self) -> np.ndarray:
+    def rows(self, cap: int | None = None) -> np.ndarray:
+        """Rows of the last pass in (group, star, cluster) order; cap: only the first cap of them."""
         n = _L(0)
         _check(self._L.phx_rows(self.h, None, 0, C.byref(n)), "phx_rows")
-        out = np.zeros(n.value, ROW_DTYPE)
-        if n.value:
-            _check(self._L.phx_rows(self.h, _p(out), n.value, C.byref(n)), "phx_rows")
+        m = n.value if cap is None else min(cap, n.value)
+        out = np.zeros(m, ROW_DTYPE)
+        if m:
+            _check(self._L.phx_rows(self.h, _p(out), m, C.byref(n)), "phx_rows")
         return out
 
     def screen(self, form: int = PLAIN, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> np.ndarray:
@@ -346,3 +370,17 @@ class Engine:
         _check(self._L.phx_pair_ex(self.h, form, DRAW_ON_DEVICE if device_draws else 0, _p(targets), _p(clusters),
                                    len(ti), _p(ti), _p(ci), n_clones, dt_myr, _p(res)), "phx_pair_ex")
         return res
+
+    def pair_loaded(self, star, cluster, n_clones: int, dt_myr: float = 0.05, form: int = PLAIN) -> np.ndarray:
+        """Monte Carlo pairs of the loaded stars and resident clusters with device draws (phx_pair_loaded)."""
+        star = np.ascontiguousarray(star, dtype=np.int32)
+        cluster = np.ascontiguousarray(cluster, dtype=np.int32)
+        res = np.zeros(len(star), RESULT_DTYPE)
+        _check(self._L.phx_pair_loaded(self.h, form, len(star), _p(star), _p(cluster), n_clones, dt_myr, _p(res)),
+               "phx_pair_loaded")
+        return res
+
+    def pair_graph_stats(self) -> tuple[int, int]:
+        c, r = _L(0), _L(0)
+        _check(self._L.phx_pair_graph_stats(self.h, C.byref(c), C.byref(r)), "phx_pair_graph_stats")
+        return c.value, r.value

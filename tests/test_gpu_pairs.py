@@ -1,0 +1,118 @@
+"""The Monte Carlo pair search on the device (pairs.hip) against the oracle.
+
+* More than 4,096 clones per pair (the reduction then selects the quantiles' order statistics by radix
+  passes instead of sorting in LDS): 5,000 clones of one pair, host draws, against ph_pair of the oracle.
+* Pairs of the loaded stars and resident clusters (phx_pair_loaded, device draws from the resident records)
+  give the same results as the same pairs with host-given records (phx_pair_ex, device draws).
+* Graph replays of a batch equal its eager run.
+* The two-stage shard on a reduced sample (600 stars of seed 4): stage 1 screens, stage 2 runs the exact Monte
+  Carlo with 1,000 clones per pair for every row with p_r50 >= 0.01; against the oracle's ph_pair on the same
+  pairs.  Bar: quantiles within 1e-3 relative (the Monte Carlo bar; the plain form's measured maxima go to
+  gpurun_out/parity_pairs.json), f_r50 / f_rtot identical.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from phygas_amd import hip
+from phygas_amd import io as pio
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INP = os.path.join(ROOT, "tests", "golden", "inputs")
+QF = ("d05", "d50", "d95", "t05", "t50", "t95", "v05", "v50", "v95")
+REPORT = {}
+
+
+def _report(key, value):
+    REPORT[key] = float(value)
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", "parity_pairs.json"), "w") as f:
+        json.dump(REPORT, f, indent=1)
+
+
+def _qrel(a, b):
+    return max(float(np.max(np.abs(a[f] - b[f]) / np.maximum(np.abs(b[f]), 1e-300))) for f in QF)
+
+
+def test_more_clones_than_the_lds_sort_takes(oracle):
+    cl = pio.read_clusters(os.path.join(INP, "clusters400.txt"))
+    pul = pio.read_targets(os.path.join(INP, "pulsar1.txt"))
+    i = int(np.argmax(cl["tmax_myr"] < 20))          # a short trace-back keeps the CPU side quick
+    ti, ci = np.array([0, 0], np.int32), np.array([i, i + 1], np.int32)
+    want = oracle.pair_run(pul, cl, ti, ci, n_clones=5000, dt_myr=0.05)
+    with hip.Engine(0) as e:
+        got = e.pair(pul, cl, ti, ci, n_clones=5000, dt_myr=0.05, form=hip.PLAIN)
+        dev = e.pair(pul, cl, ti, ci, n_clones=5000, dt_myr=0.05, form=hip.PLAIN, device_draws=True)
+    assert np.array_equal(got["n"], want["n"]) and got["n"][0] == 5000
+    assert np.array_equal(got["f_r50"], want["f_r50"]) and np.array_equal(got["f_rtot"], want["f_rtot"])
+    _report("clones5000_host_draws_rel", _qrel(got, want))
+    _report("clones5000_device_draws_rel", _qrel(dev, want))
+    assert _qrel(got, want) <= 1e-9
+    assert _qrel(dev, want) <= 1e-9
+
+
+def test_loaded_pairs_equal_host_record_pairs(workload):
+    cl = workload.make_clusters(600, 0)
+    st = workload.make_stars(40, 4)
+    rng = np.random.default_rng(9)
+    si = rng.integers(0, len(st), 64).astype(np.int32)
+    ci = rng.integers(0, len(cl), 64).astype(np.int32)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        for form in (hip.PLAIN, hip.FAST):
+            a = e.pair_loaded(si, ci, n_clones=300, form=form)
+            b = e.pair(st, cl, si, ci, n_clones=300, form=form, device_draws=True)
+            assert a.tobytes() == b.tobytes(), form
+
+
+def test_graph_replays_equal_eager(workload):
+    cl = workload.make_clusters(600, 0)
+    st = workload.make_stars(30, 4)
+    rng = np.random.default_rng(3)
+    si = rng.integers(0, len(st), 48).astype(np.int32)
+    ci = rng.integers(0, len(cl), 48).astype(np.int32)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        e.set_graphs(False)
+        eager = e.pair_loaded(si, ci, n_clones=200, form=hip.FAST)
+        other = e.pair_loaded(si[::-1].copy(), ci, n_clones=200, form=hip.FAST)
+        e.set_graphs(True)
+        outs = []
+        for rep in range(4):
+            outs.append(e.pair_loaded(si, ci, n_clones=200, form=hip.FAST))
+            e.sync()
+            assert e.pair_loaded(si[::-1].copy(), ci, n_clones=200, form=hip.FAST).tobytes() == other.tobytes()
+        cap, rep_ = e.pair_graph_stats()
+    for o in outs:
+        assert o.tobytes() == eager.tobytes()
+    assert cap == 1 and rep_ >= 7
+
+
+def test_two_stage_shard_against_the_oracle(workload, oracle):
+    cl = workload.make_clusters(4087, 0)
+    st = workload.make_stars(600, 4)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        rows = e.screen(hip.FAST, chi2_max=25.0, p_min=0.01)
+        assert len(rows) >= 15
+        si, ci = rows["star"].astype(np.int32), rows["cluster"].astype(np.int32)
+        plain = e.pair_loaded(si, ci, n_clones=1000, form=hip.PLAIN)
+        fast = e.pair_loaded(si, ci, n_clones=1000, form=hip.FAST)
+    want = oracle.pair_run(st, cl, si, ci, n_clones=1000, dt_myr=0.05, nthreads=min(16, os.cpu_count() or 1))
+    _report("two_stage_pairs", len(si))
+    _report("two_stage_plain_rel", _qrel(plain, want))
+    _report("two_stage_fast_rel", _qrel(fast, want))
+    assert np.array_equal(plain["n"], want["n"])
+    assert np.array_equal(plain["f_r50"], want["f_r50"]) and np.array_equal(plain["f_rtot"], want["f_rtot"])
+    assert _qrel(plain, want) <= 1e-9
+    assert _qrel(fast, want) <= 1e-3
+    df = np.maximum(np.abs(fast["f_r50"] - want["f_r50"]), np.abs(fast["f_rtot"] - want["f_rtot"]))
+    _report("two_stage_fast_f_max_diff", df.max())
+    assert df.max() <= 2.0 / 1000      # at most a clone or two on a radius

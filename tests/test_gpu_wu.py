@@ -61,13 +61,38 @@ def test_screen_work_unit_rows_match_oracle(oracle, tmp_path):
     assert got == want
 
 
+def test_default_options_give_the_oracle_rows(oracle, tmp_path):
+    """With phx_wu_defaults (FAST64 screening, FP64 clones) the app's rows are the reference's."""
+    o = hip.wu_defaults()
+    assert (o.form, o.pair_form) == (hip.FAST64, hip.PLAIN)
+    inp, out, ck = str(tmp_path / "in"), str(tmp_path / "out"), str(tmp_path / "ckpt")
+    _write_in(inp, "PHYGAS 1 screen 25 64 2 0", _star_lines())
+    assert hip.wu_run(inp, CLUSTERS, out, ck, batch_targets=7) == (hip.WU_DONE, 20)
+    tg, cl = pio.read_targets(os.path.join(INP, "stars20.txt")), pio.read_clusters(CLUSTERS)
+    want = _screen_expected(oracle, tg, cl, 0.0)
+    assert len(want) > 5 and _lines(out) == want
+    with open(os.path.join(INP, "pulsar1.txt")) as f:
+        pulsar = [ln for ln in f if ln.strip()][0]
+    pin, pout, pck = str(tmp_path / "pin"), str(tmp_path / "pout"), str(tmp_path / "pckpt")
+    _write_in(pin, "PHYGAS 1 pair 40 0.05 0 0", [pulsar])
+    assert hip.wu_run(pin, CLUSTERS, pout, pck) == (hip.WU_DONE, 1)
+    pt = pio.read_targets(os.path.join(INP, "pulsar1.txt"))[:1]
+    mask = oracle.prefilter_mask(pt, cl)[0]
+    ci = list(np.nonzero(mask)[0])
+    res = oracle.pair_run(pt, cl, [0] * len(ci), ci, 40, 0.05)
+    want = [f"{_s(pt[0]['id'])} {_s(cl[i]['name'])} {_s(cl[i]['kind'])} {r['n']} {r['d05']:.3f} {r['d50']:.3f} "
+            f"{r['d95']:.3f} {r['t05']:.4f} {r['t50']:.4f} {r['t95']:.4f} {r['v05']:.2f} {r['v50']:.2f} "
+            f"{r['v95']:.2f} {r['f_r50']:.4f} {r['f_rtot']:.4f}\n" for i, r in zip(ci, res)] + ["# done 1\n"]
+    assert len(want) > 10 and _lines(pout) == want
+
+
 def test_resumed_run_is_identical(tmp_path):
     inp = str(tmp_path / "in")
     _write_in(inp, "PHYGAS 1 screen 25 64 2 0", _star_lines())
     ref_out, ref_ck = str(tmp_path / "ref_out"), str(tmp_path / "ref_ckpt")
     assert hip.wu_run(inp, CLUSTERS, ref_out, ref_ck, form=hip.FAST, batch_targets=4) == (0, 20)
     out, ck = str(tmp_path / "out"), str(tmp_path / "ckpt")
-    assert hip.wu_run(inp, CLUSTERS, out, ck, form=hip.FAST, batch_targets=4, max_batches=2) == (1, 8)
+    assert hip.wu_run(inp, CLUSTERS, out, ck, form=hip.FAST, batch_targets=4, max_batches=2) == (hip.WU_STOPPED, 8)
     with open(out, "a") as f:
         f.write("S0000009 C0001 o 12.3")              # a row torn by the interruption
     assert hip.wu_run(inp, CLUSTERS, out, ck, form=hip.FAST, batch_targets=4) == (0, 20)
@@ -87,7 +112,7 @@ def test_pair_work_unit_rows_match_oracle(oracle, tmp_path):
     star = _star_lines()[3].rstrip("\n") + " clusters=" + ",".join(names)
     inp, out, ck = str(tmp_path / "in"), str(tmp_path / "out"), str(tmp_path / "ckpt")
     _write_in(inp, "PHYGAS 1 pair 40 0.05 0 0", [pulsar, star])
-    rc, done = hip.wu_run(inp, CLUSTERS, out, ck, form=hip.PLAIN, batch_targets=2)
+    rc, done = hip.wu_run(inp, CLUSTERS, out, ck, pair_form=hip.PLAIN, batch_targets=2)
     assert rc == 0 and done == 2
     tg = np.concatenate([pio.read_targets(os.path.join(INP, "pulsar1.txt"))[:1],
                          pio.read_targets(os.path.join(INP, "stars20.txt"))[3:4]])

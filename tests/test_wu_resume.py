@@ -58,3 +58,16 @@ def test_unreadable_checkpoint_leaves_everything(tmp_path):
     rc, t, n = hip.wu_resume(out, ck, 10)
     assert rc == 4 and (t, n) == (0, 0)
     assert _read(out) == ROWS
+
+
+def test_status_codes_are_distinct(tmp_path):
+    """Stopped, unreadable checkpoint and error are three different answers; an error raises."""
+    assert len({hip.WU_DONE, hip.WU_STOPPED, hip.WU_CKPT_UNREADABLE, 1, 2}) == 5
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                            "phygas_hip.h")).read()
+    assert "PHX_WU_CKPT_UNREADABLE = 4, PHX_WU_STOPPED = 6" in hdr
+    # a missing cluster file fails before any device work: the code is an error, and the wrapper raises
+    inp = str(tmp_path / "in")
+    _write(inp, "PHYGAS 1 screen 25 64 2 0\n")
+    with pytest.raises(hip.PhygasError):
+        hip.wu_run(inp, str(tmp_path / "no_such_clusters"), str(tmp_path / "out"), str(tmp_path / "ck"))

@@ -232,6 +232,19 @@ int phx_wu_resume(const char *out_path, const char *ckpt_path, long nt, long *ta
  * src/phygas_cli.c:103-117 runs them): mask[j * nc + i] = 1 when target j may have reached cluster i. */
 int phx_prefilter(PhxCtx *ctx, const PhTarget *t, int nt, const PhCluster *c, int nc, unsigned char *mask);
 
+/* ---------------------------------------------------------------- 5. allocation-free Monte Carlo batch
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc): phx_pair_ex with PHX_DRAW_ON_DEVICE on the
+ * current device, without a context, calling no allocation routine.  The batch's buffers come from a static
+ * device array of phx_pair_noalloc_limit() bytes per GPU (64 MiB, held until the process ends); a batch that
+ * needs more (about 132 bytes per clone slot plus the records) is rejected with code 2 before any device work.
+ * Every call on one GPU shares that array, which keeps the last call's contents: calls that overlap on one GPU
+ * (from several threads or streams) race.  stream: a hipStream_t (null: the default stream); the call returns
+ * once the results are in res.  Results are those of phx_pair_ex. */
+size_t phx_pair_noalloc_limit(void);
+int phx_pair_noalloc(int form, const PhTarget *t, const PhCluster *c, int npairs, const int *ti, const int *ci,
+                     int n_clones, double dt_myr, PhResult *res, void *stream);
+
 /* Host helpers with the reference's semantics. */
 uint64_t phx_seed(const char *a, const char *b); /* reference: src/phygas.c:452-458 */
 int phx_pair_draw(const PhTarget *t, const PhCluster *c, int n_clones, double dt_myr, uint64_t seed, double *ps,

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -471,3 +472,106 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
 }
 
 }  // namespace phx
+
+// ================================================================ allocation-free Monte Carlo batch
+// For callers that cannot allocate device memory (code under stream capture, tight serving loops, harnesses
+// that forbid hipMalloc): the batch's buffers come from `scratch`, a static device array of PAIR_SCRATCH_BYTES
+// per GPU (held until the process ends).  In host code the array's name is a stand-in, not a device address:
+// its address on each device is taken once, with that device current, by hipGetSymbolAddress and kept in
+// g_scratch[device]; only that pointer is handed to kernels and copies.
+namespace phx {
+
+constexpr size_t PAIR_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[PAIR_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static char *g_scratch[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(int device, char **out) {
+    PHX_REQUIRE(device >= 0 && device < SCRATCH_MAX_DEVICES, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        PHX_CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PHX_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch[device] = (char *)p;
+    }
+    *out = g_scratch[device];
+    return 0;
+}
+
+// Per-pair plan on the device, in the caller's pair order: seed ph_seed(id, name) (FNV over "id|name"),
+// steps ceil(tmax / dt_myr), step -tmax / steps (reference: src/phygas.c:185-187, 452-458), radii, and
+// n_clones slots per pair.
+__global__ __launch_bounds__(256) void k_pair_plan(const PhTarget *__restrict__ tg, const PhCluster *__restrict__ cl,
+                                                   const int *__restrict__ ti, const int *__restrict__ ci, int npairs,
+                                                   int n_clones, double dt_myr, uint64_t *__restrict__ seed,
+                                                   int *__restrict__ psteps, double *__restrict__ pdt,
+                                                   double *__restrict__ r50, double *__restrict__ rt,
+                                                   long *__restrict__ off) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const PhTarget &T = tg[ti[p]];
+    const PhCluster &C = cl[ci[p]];
+    seed[p] = fnv_feed(seed_prefix(T.id), C.name, 40);
+    const int s = (int)ceil(C.tmax_myr / dt_myr);
+    psteps[p] = s;
+    pdt[p] = -C.tmax_myr / s;
+    r50[p] = C.r50_pc;
+    rt[p] = C.rtot_pc;
+    off[p] = (long)p * n_clones;
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" size_t phx_pair_noalloc_limit(void) { return PAIR_SCRATCH_BYTES; }
+
+extern "C" int phx_pair_noalloc(int form, const PhTarget *t, const PhCluster *c, int npairs, const int *ti,
+                                const int *ci, int n_clones, double dt_myr, PhResult *res, void *stream) {
+    PHX_REQUIRE(npairs >= 0 && n_clones >= 0, "negative size");
+    PHX_REQUIRE(form == PHX_FORM_PLAIN || form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "unknown form");
+    PHX_REQUIRE(dt_myr > 0, "dt must be positive");
+    if (form == PHX_FORM_FAST64) form = PHX_FORM_PLAIN;
+    if (npairs == 0) return 0;
+    int nt = 0, ncl = 0;
+    for (int p = 0; p < npairs; p++) {
+        PHX_REQUIRE(ti[p] >= 0 && ci[p] >= 0, "negative record index");
+        nt = std::max(nt, ti[p] + 1);
+        ncl = std::max(ncl, ci[p] + 1);
+    }
+    const long slots = (long)npairs * n_clones;
+    PHX_REQUIRE((double)npairs * n_clones < 2.0e9, "too many clones in one batch");
+    const PairLayout sz = pair_layout(nullptr, slots, npairs, nt, ncl);
+    if (sz.total > PAIR_SCRATCH_BYTES) {
+        char msg[192];
+        snprintf(msg, sizeof msg, "batch needs %zu bytes of scratch, above the %zu-byte limit (phx_pair_noalloc_limit)",
+                 sz.total, PAIR_SCRATCH_BYTES);
+        set_error("phx_pair_noalloc", msg);
+        return 2;
+    }
+    int dev = 0;
+    PHX_CK(hipGetDevice(&dev));
+    char *base = nullptr;
+    if (scratch_ptr(dev, &base)) return 1;
+    const PairLayout L = pair_layout(base, slots, npairs, nt, ncl);
+    hipStream_t st = (hipStream_t)stream;
+    const Pot P = default_pot();
+    PHX_CK(hipMemcpyAsync(L.tg, t, sizeof(PhTarget) * (size_t)nt, hipMemcpyHostToDevice, st));
+    PHX_CK(hipMemcpyAsync(L.cl, c, sizeof(PhCluster) * (size_t)ncl, hipMemcpyHostToDevice, st));
+    PHX_CK(hipMemcpyAsync(L.ti, ti, 4 * (size_t)npairs, hipMemcpyHostToDevice, st));
+    PHX_CK(hipMemcpyAsync(L.ci, ci, 4 * (size_t)npairs, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_pair_plan, dim3(blocks_for(npairs, 256)), dim3(256), 0, st, L.tg, L.cl, L.ti, L.ci, npairs,
+                       n_clones, dt_myr, L.seed, L.psteps, L.pdt, L.r50, L.rt, L.off);
+    PHX_CK(hipGetLastError());
+    PairPlan plan;   // sizes only: the per-pair arrays were built on the device
+    plan.npairs = npairs;
+    plan.n_clones = n_clones;
+    plan.slots = slots;
+    if (pair_launch_device(L, plan, form, L.tg, L.cl, P, st)) return 1;
+    PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, st));
+    PHX_CK(hipStreamSynchronize(st));
+    return 0;
+}

@@ -77,6 +77,8 @@ def lib() -> C.CDLL:
         "phx_check_crmath": (_I, [_I, _I, _P, _P, _L, _P]),
         "phx_pair_ex": (_I, [_P, _I, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
         "phx_pair_loaded": (_I, [_P, _I, _I, _P, _P, _I, _D, _P]),
+        "phx_pair_noalloc": (_I, [_I, _P, _P, _I, _P, _P, _I, _D, _P, _P]),
+        "phx_pair_noalloc_limit": (C.c_size_t, []),
         "phx_pair_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
@@ -118,6 +120,20 @@ def pair_draw(t, c, n_clones: int, dt_myr: float, seed_: int):
     dt = np.zeros(1)
     m = lib().phx_pair_draw(_p(t), _p(c), n_clones, dt_myr, seed_, _p(ps), _p(steps), _p(dt))
     return ps[:m], int(steps[0]), float(dt[0])
+
+
+def pair_noalloc(targets, clusters, ti, ci, n_clones: int, dt_myr: float = 0.05, form: int = PLAIN,
+                 stream: int | None = None) -> np.ndarray:
+    """The allocation-free Monte Carlo batch (phx_pair_noalloc): device draws on the current device, buffers
+    from the library's static scratch array."""
+    targets = np.ascontiguousarray(np.atleast_1d(targets), dtype=TARGET_DTYPE)
+    clusters = np.ascontiguousarray(np.atleast_1d(clusters), dtype=CLUSTER_DTYPE)
+    ti = np.ascontiguousarray(ti, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    res = np.zeros(len(ti), RESULT_DTYPE)
+    _check(lib().phx_pair_noalloc(form, _p(targets), _p(clusters), len(ti), _p(ti), _p(ci), n_clones, dt_myr, _p(res),
+                                  stream), "phx_pair_noalloc")
+    return res
 
 
 CRMATH_FUNCS = {"pow": 0, "pow15": 1, "hypot": 2, "sin": 3, "cos": 4}

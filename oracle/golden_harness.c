@@ -3,7 +3,10 @@
  * One source, built twice by oracle/Makefile: against the compiled reference (-DUSE_REF, headers from
  * the reference's src/) into oracle/_ref/golden_ref, and against the oracle (oracle/liboracle.so) into
  * oracle/golden_orc.  Both print the same records as hex floats; tests/golden/ holds the reference's
- * output and tests/test_oracle.py requires the oracle's to be identical, byte for byte.
+ * output and tests/test_oracle.py requires the oracle's to be identical, byte for byte.  tests/test_capi.py
+ * builds it a third time (-DUSE_CORE) against the product's core API (include/phygas_core.h,
+ * phygas_amd/libphygas_hip.so), whose output must be identical too (every section but coords, which is not
+ * part of that API).
  *
  *   golden_harness <section> clusters.txt targets.txt
  * sections: accel rng seed coords sample lininit lin chi2 pair track read */
@@ -26,6 +29,14 @@ static void from_gc(const double ps[6], double o[6]) {
     o[0] = a.ra_deg; o[1] = a.dec_deg; o[2] = a.parallax_mas; o[3] = a.pmra_masyr; o[4] = a.pmdec_masyr;
     o[5] = a.rv_kms;
 }
+#elif defined(USE_CORE)
+#include "phygas_core.h"
+static void to_gc(double ra, double dec, double plx, double pmra, double pmdec, double rv, double o[6]) {
+    (void)ra; (void)dec; (void)plx; (void)pmra; (void)pmdec; (void)rv; (void)o;
+    fprintf(stderr, "coords: not part of the core API\n");
+    exit(2);
+}
+static void from_gc(const double ps[6], double o[6]) { (void)ps; (void)o; exit(2); }
 #else
 #include "phygas_oracle.h"
 static void to_gc(double ra, double dec, double plx, double pmra, double pmdec, double rv, double o[6]) {

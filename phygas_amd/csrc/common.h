@@ -6,6 +6,9 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <string>
+#include <vector>
+
 #include "physics.h"
 #include "phygas_hip.h"
 
@@ -79,6 +82,14 @@ int launch_rows(const long *d_pair_idx, const long *d_ridx, const long *d_rcount
 int launch_prefilter(const PhTarget *d_t, const uint64_t *d_tseed, const int *d_tn, int nt, const PhCluster *d_c,
                      const uint64_t *d_cseed, const int *d_cn, int nc, int npts, const Pot &P, double *d_tpath,
                      double *d_tspr, double *d_cpath, double *d_cspr, unsigned char *d_mask, hipStream_t st);
+// ---- record files (wu.hip)
+// Cluster line (19 fields) and target line (14 fields, then an optional clusters=a,b,... list into *only), with
+// the reference's field widths (reference: src/phygas.c:460-506); false for a line that does not parse.
+bool parse_cluster(const char *line, PhCluster *c);
+bool parse_target(const char *line, PhTarget *t, std::string *only);
+// Every line of a file (newlines kept; a last line without one too); false when it cannot be opened.
+bool read_lines(const char *path, std::vector<std::string> &lines);
+
 // ---- device-wide selection (select.hip) ---------------------------------------------------------
 // Indices i in [0, n) with val[i] < thr (strictly; NaN never), in increasing order, into d_idx[0 .. cap);
 // the full count into *d_count (device; may exceed cap, then only cap indices are written).

@@ -25,6 +25,7 @@
 
 #include "common.h"
 
+namespace phx {
 namespace {
 
 // Whitespace tokens with sscanf-like field widths: a %Ns field takes at most N characters and the rest of the
@@ -67,6 +68,8 @@ bool num(const std::string &s, double *v, bool nan_ok) {
 void copy_str(char *dst, size_t cap, const std::string &s) {
     snprintf(dst, cap, "%s", s.c_str());
 }
+
+}  // namespace
 
 // Cluster line: index name kind ra dec dist e_dist pmra e_pmra pmdec e_pmdec rv e_rv r50 rtot age16 age50 age84
 // tmax (reference: src/phygas.c:462-485).  Lines that do not parse are skipped.
@@ -116,6 +119,8 @@ bool parse_target(const char *line, PhTarget *t, std::string *only) {
     return true;
 }
 
+namespace {
+
 // name is one element of the comma-separated list
 bool in_list(const char *name, const std::string &list) {
     size_t a = 0;
@@ -127,6 +132,8 @@ bool in_list(const char *name, const std::string &list) {
     }
     return false;
 }
+
+}  // namespace
 
 bool read_lines(const char *path, std::vector<std::string> &lines) {
     FILE *f = fopen(path, "rb");
@@ -144,6 +151,8 @@ bool read_lines(const char *path, std::vector<std::string> &lines) {
     fclose(f);
     return true;
 }
+
+namespace {
 
 bool file_exists(const char *path) {
     struct stat st;
@@ -166,6 +175,7 @@ bool write_atomic(const char *path, const std::string &data) {
 }
 
 }  // namespace
+}  // namespace phx
 
 using namespace phx;
 

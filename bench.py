@@ -1,12 +1,22 @@
-"""Benchmark of the screening work unit on one MI355X: 20,000 synthetic stars x 4,087 synthetic clusters with
-the fused shuffled-velocity control (stars seed 2, clusters seed 0, control permutation seed 3), rows with
+"""Benchmark of the trace-back path on one MI355X.
+
+Headline (value): the screening work unit, 20,000 synthetic stars x 4,087 synthetic clusters with the fused
+shuffled-velocity control (stars seed 2, clusters seed 0, control permutation seed 3), rows with
 p_r50 >= 0.001.  A step is one device pass over the whole unit (draws, tracks, scan, finish, row compaction)
-with the cluster tables resident and the star records already in HBM.
+with the cluster tables resident and the star records already in HBM.  The plain FP64 form is timed beside it
+and the rows of the two compared.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fast|plain] [--stars N] [--no-cpu]
+Further sections of the same JSON line:
+  monte_carlo  1 pulsar x 300 prefiltered clusters x 200 clones, plain FP64 and FP32 clones, device draws
+  two_stage    a 100,000-star shard (seed 4) with its control: the fast screening pass, then the Monte Carlo
+               (1,000 clones, device draws, graph replay) of every real row with p_r50 >= 0.01
+  serving      the request mix at swept Poisson rates, fast and plain forms (p50/p95/p99, pairs/s)
+  cpu_baseline the reference's own code on the host: 1 thread and N threads, interleaved runs, medians
 
-Prints one JSON line.  value = pairs screened per second (real + control pairs) of the chosen form;
-the plain FP64 form is timed beside it (plain_value, speedup) and their p_r50 agreement is checked.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fast|plain] [--stars N]
+                    [--no-cpu] [--no-plain] [--no-mc] [--no-two-stage] [--no-serve]
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -38,6 +48,9 @@ PLAIN_LAST_FLOP = 56.0
 SCAN_L1_FLOP = 76.0        # cull.h l1_pass, one (pair, tile) test, FP32
 SCAN_L2_FLOP = 367.0       # cull.h l2f_pass, one (pair, tile) certificate, FP32 (both quadratic checks, vertex)
 POINT_FLOP = 134.0         # k_items, one pair-point: both Mahalanobis forms and the segment parameter, FP64
+# Monte Carlo, per clone-step (two bodies, one kick-drift-kick each, then the step segment's closest approach):
+PLAIN_CLONE_STEP_FLOP = 192.0   # FP64: 2 x (58 force (each pow counted once) + 20 kick-drift-kick) + 36 segment
+FAST_CLONE_STEP_FLOP = 154.0    # 2 x 41 FP32 force + 2 x 18 FP64 kick-drift-kick + 36 FP64 segment
 
 
 def plain_scan_flops(cn: np.ndarray, n_stars: int) -> float:
@@ -56,25 +69,54 @@ def pmc_traffic(kernel: str) -> float | None:
         return float(json.load(fh)["traffic_bytes"])
 
 
-def cpu_baseline(clusters, stars, threads: int) -> dict | None:
-    """The reference's own screening loop (oracle/_ref, built from the reference sources) or, without it, the
-    oracle restatement, on a bounded sample: about 50 stars per thread against every cluster."""
+def _cpu_exe():
     ref = os.path.join(ROOT, "oracle", "_ref", "cpu_bench_ref")
     port = os.path.join(ROOT, "oracle", "cpu_bench_orc")
-    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
-    if not os.path.exists(exe):
+    if os.path.exists(ref):
+        return ref, "reference"
+    return (port, "port") if os.path.exists(port) else (None, None)
+
+
+def _kv(line: str) -> dict:
+    return {k: (float(v) if any(ch.isdigit() for ch in v) and k != "mode" else v)
+            for k, v in (x.split("=") for x in line.split())}
+
+
+def cpu_baseline(clusters, stars, pulsar, threads: int, reps: int = 3) -> dict | None:
+    """The reference's own code (oracle/_ref, built from the reference sources) or, without it, the oracle
+    restatement, on the GPU box's host cores: the screening loop of phygas_screen on a bounded sample (25 stars
+    per thread against every cluster, the chi2 cut and the finish) and ph_pair (5 pairs of 200 clones per
+    thread), each at 1 thread and at `threads` threads, interleaved over `reps` rounds, medians reported."""
+    exe, kind = _cpu_exe()
+    if exe is None:
         return None
-    n = 50 * threads
+    n1, nN = 25, 25 * threads
+    p1, pN = 5, 5 * threads
     with tempfile.TemporaryDirectory() as d:
-        cf, sf = os.path.join(d, "clusters.txt"), os.path.join(d, "stars.txt")
+        cf, sf, pf = (os.path.join(d, f) for f in ("clusters.txt", "stars.txt", "pulsar.txt"))
         wl.write_clusters(cf, clusters)
-        wl.write_targets(sf, stars[:n])
-        out = subprocess.run([exe, cf, sf, str(threads), str(n)], capture_output=True, text=True, check=True).stdout
-    kv = dict(x.split("=") for x in out.split())
-    return {"value": float(kv["pairs_per_s"]), "unit": "pairs/s", "cores": threads, "kind": kind,
-            "sample": f"{n} of the work unit's stars x {kv['clusters']} clusters, chi2 cut 25 + finish "
-                      f"({kv['kept']} kept pairs); cluster tracks prepared first in {float(kv['cluster_prep_s']):.2f} s "
-                      f"and not timed"}
+        wl.write_targets(sf, stars[:nN])
+        wl.write_targets(pf, pulsar)
+        scr = subprocess.run([exe, "screen", cf, sf, str(reps), f"1:{n1}", f"{threads}:{nN}"], capture_output=True,
+                             text=True, check=True).stdout.splitlines()
+        par = subprocess.run([exe, "pair", cf, pf, str(reps), "200", "0.05", f"1:{p1}", f"{threads}:{pN}"],
+                             capture_output=True, text=True, check=True).stdout.splitlines()
+    s1, sN = _kv(scr[0]), _kv(scr[1])
+    m1, mN = _kv(par[0]), _kv(par[1])
+    return {"value": sN["pairs_per_s"], "unit": "pairs/s", "cores": threads, "kind": kind,
+            "sample": f"{nN} of the work unit's stars x {int(sN['clusters'])} clusters on {threads} threads, "
+                      f"chi2 cut 25 + finish ({int(sN['kept'])} kept pairs); median of {reps} runs interleaved with "
+                      f"the 1-thread runs; cluster tracks prepared first ({sN['cluster_prep_s']:.2f} s) and not timed",
+            "screen_1_thread": {"pairs_per_s": s1["pairs_per_s"], "stars": int(s1["stars"]), "median_s": s1["median_s"],
+                                "min_s": s1["min_s"], "max_s": s1["max_s"]},
+            "screen_n_threads": {"threads": threads, "pairs_per_s": sN["pairs_per_s"], "stars": int(sN["stars"]),
+                                 "median_s": sN["median_s"], "min_s": sN["min_s"], "max_s": sN["max_s"]},
+            "pair_1_thread": {"clone_steps_per_s": m1["clone_steps_per_s"], "pairs": int(m1["pairs"]),
+                              "median_s": m1["median_s"], "min_s": m1["min_s"], "max_s": m1["max_s"]},
+            "pair_n_threads": {"threads": threads, "clone_steps_per_s": mN["clone_steps_per_s"],
+                               "pairs": int(mN["pairs"]), "median_s": mN["median_s"], "min_s": mN["min_s"],
+                               "max_s": mN["max_s"]},
+            "runs": reps}
 
 
 def copy_probe(gib: float) -> int:
@@ -125,6 +167,118 @@ def run_form(e: hip.Engine, form: int, steps: int, warmup: int, chi2_max: float,
     return ms, {k: v / max(stage_passes, 1) for k, v in stage_sum.items()}
 
 
+def _qrel(a, b) -> float:
+    fs = ("d05", "d50", "d95", "t05", "t50", "t95", "v05", "v50", "v95")
+    return max(float(np.max(np.abs(a[f] - b[f]) / np.maximum(np.abs(b[f]), 1e-300))) for f in fs)
+
+
+def _median_ms(fn, reps: int) -> float:
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        t.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(t))
+
+
+def monte_carlo(e: hip.Engine, clusters, pulsar, reps: int = 5) -> dict:
+    """1 pulsar x 300 prefiltered clusters x 200 clones: the plain FP64 and the FP32 clone forms, clones drawn on
+    the device (every result checked against the plain form), timed per call with the host round trip, and the
+    clone kernel alone from stage events."""
+    mask = e.prefilter(pulsar, clusters)[0]
+    ci = np.nonzero(mask)[0]
+    if len(ci) < 300:
+        ci = np.concatenate([ci, np.setdiff1d(np.arange(len(clusters)), ci)[:300 - len(ci)]])
+    ci = np.sort(ci[:300]).astype(np.int32)
+    ti = np.zeros(len(ci), np.int32)
+    n_clones, dt = 200, 0.05
+    steps = float(np.sum(np.ceil(clusters["tmax_myr"][ci] / dt))) * n_clones
+    out = {"workload": "1 pulsar x 300 prefiltered clusters x 200 clones, dt 0.05 Myr, device draws",
+           "clone_steps": steps}
+    res = {}
+    for name, form, flop in (("plain", hip.PLAIN, PLAIN_CLONE_STEP_FLOP), ("fast", hip.FAST, FAST_CLONE_STEP_FLOP)):
+        res[name] = e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True)
+        ms = _median_ms(lambda: e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True), reps)
+        e.set_timing(True)
+        e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True)
+        st = e.stage_times()
+        e.set_timing(False)
+        kms = st.get("clones", float("nan"))
+        achieved = steps * flop / (kms / 1e3) / 1e12
+        out[name] = {"ms_per_job": ms, "clone_steps_per_s": steps / (ms / 1e3), "stages_ms": st,
+                     "clone_kernel_clone_steps_per_s": steps / (kms / 1e3),
+                     "roofline": {"bound": "valu", "kernel": "k_clones<%d>" % int(form == hip.FAST),
+                                  "achieved": achieved, "peak": PEAK_VALU, "unit": "TFLOP/s",
+                                  "frac": achieved / PEAK_VALU, "work_per_launch": steps * flop,
+                                  "flop_per_clone_step": flop, "avg_launch_ms": kms}}
+    host = e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=hip.PLAIN, device_draws=False)
+    out["fast_vs_plain_quantile_rel_max"] = _qrel(res["fast"], res["plain"])
+    out["fast_vs_plain_f_max_abs"] = float(max(np.max(np.abs(res["fast"]["f_r50"] - res["plain"]["f_r50"])),
+                                               np.max(np.abs(res["fast"]["f_rtot"] - res["plain"]["f_rtot"]))))
+    out["device_vs_host_draws_quantile_rel_max"] = _qrel(res["plain"], host)
+    out["speedup_fast_vs_plain"] = out["plain"]["ms_per_job"] / out["fast"]["ms_per_job"]
+    return out
+
+
+def two_stage(clusters, n_stars: int = 100000, n_clones: int = 1000, p_stage2: float = 0.01) -> dict:
+    """A 100,000-star shard (seed 4) with its shuffled control: stage 1 the fast screening pass, stage 2 the
+    Monte Carlo of every real row with p_r50 >= 0.01 (device draws, FP32 clones, the batch's launches replayed
+    from a hipGraph), the plain FP64 clones timed beside it."""
+    stars = wl.make_stars(n_stars, 4)
+    control = wl.shuffled_control(stars, 3)
+    out = {"workload": f"{n_stars:,} stars (seed 4) + shuffled control x {len(clusters)} clusters; stage 2: real rows "
+                       f"with p_r50 >= {p_stage2}, {n_clones} clones each, dt 0.05 Myr"}
+    with hip.Engine(0) as e:
+        e.load_clusters(clusters, 64, 2.0)
+        e.load_stars(stars, 0)
+        e.load_stars(control, 1)
+        e.screen_async(hip.FAST, 25.0, 64, 0.001)
+        e.kept()
+        rows = e.rows()
+        t0 = time.perf_counter()
+        e.screen_async(hip.FAST, 25.0, 64, 0.001)
+        e.kept()
+        rows = e.rows()
+        s1 = (time.perf_counter() - t0) * 1e3
+        sel = rows[(rows["flags"] == 0) & (rows["p_r50"] >= p_stage2)]
+        si, ci = sel["star"].astype(np.int32), sel["cluster"].astype(np.int32)
+        steps = float(np.sum(np.ceil(clusters["tmax_myr"][ci] / 0.05))) * n_clones
+        e.set_graphs(True)
+        res = {}
+        for name, form in (("fast", hip.FAST), ("plain", hip.PLAIN)):
+            res[name] = e.pair_loaded(si, ci, n_clones, 0.05, form)
+            ms = _median_ms(lambda: e.pair_loaded(si, ci, n_clones, 0.05, form), 3)
+            out[f"stage2_{name}_ms"] = ms
+            out[f"stage2_{name}_clone_steps_per_s"] = steps / (ms / 1e3)
+        cap, rep = e.pair_graph_stats()
+    out.update({"stage1_ms": s1, "stage1_pairs": 2 * n_stars * len(clusters),
+                "stage1_pairs_per_s": 2 * n_stars * len(clusters) / (s1 / 1e3), "rows": int(len(rows)),
+                "stage2_pairs": int(len(si)), "stage2_clone_steps": steps,
+                "total_ms_fast": s1 + out["stage2_fast_ms"], "graph_captures": cap, "graph_replays": rep,
+                "stage2_fast_vs_plain_quantile_rel_max": _qrel(res["fast"], res["plain"]) if len(si) else 0.0})
+    return out
+
+
+def serving_sweep(clusters, p_min: float, rates_fast, rates_plain, n_req: int) -> dict:
+    """The serving mix (70% screening chunks of 256..20,000 stars, log-uniform; 30% pulsar jobs of 100..600
+    clusters x 1,000 clones; seed 5) at swept Poisson rates, for the fast and the plain form: latency
+    percentiles, those of the large chunks (>= 10,000 stars), pairs/s and clones/s per rate."""
+    from phygas_amd import serve
+    base = serve.request_mix(n_req, 1.0, seed=5, n_clusters=len(clusters))   # arrivals at 1 Hz, rescaled per rate
+    out = {"requests_per_rate": n_req, "mix": "70% screening chunks of 256..20,000 stars (log-uniform), 30% pulsar "
+           "jobs of 100..600 clusters x 1,000 clones; Poisson arrivals, seed 5", "fast": [], "plain": []}
+    for name, form, rates in (("fast", hip.FAST, rates_fast), ("plain", hip.PLAIN, rates_plain)):
+        with serve.Server(clusters, form=form, screen_slots=2, pair_slots=1, p_min=p_min) as srv:
+            for rate in rates:
+                mix = [(t / rate, kind, payload) for t, kind, payload in base]
+                r = serve.run_mix(srv, mix)
+                r["rate_hz"] = rate
+                out[name].append(r)
+        ok = [r["rate_hz"] for r in out[name] if r.get("large_screen_p99_s") is not None and r["large_screen_p99_s"] <= 2.0]
+        out[f"{name}_max_rate_large_chunk_p99_le_2s"] = max(ok) if ok else None
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,12 +289,16 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=150)
-    ap.add_argument("--serve-rate", type=float, default=15.0)
+    ap.add_argument("--no-mc", action="store_true")
+    ap.add_argument("--no-two-stage", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-rates-fast", default="15,60,240,960")
+    ap.add_argument("--serve-rates-plain", default="15,30,60")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing (the library's default "
                     "grids are its choices on this unit)")
     ap.add_argument("--split", type=int, default=0, help="sub-passes of a fast pass on their own streams "
-                    "(0: the library's default, 2)")
+                    "(0: the library's default, 3)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB device to device (the HBM counter calibration of the profiles) and exit")
     args = ap.parse_args()
@@ -155,6 +313,7 @@ def main() -> int:
     clusters = wl.make_clusters(4087, 0)
     stars = wl.make_stars(args.stars, 2)
     control = wl.shuffled_control(stars, 3)
+    pulsar = wl.make_pulsars(1, 1)
     chi2_max, p_min = 25.0, 0.001
     cn = (np.ceil(clusters["tmax_myr"] / 0.5) + 1).astype(np.int64)
     n_stars = 2 * len(stars)
@@ -218,24 +377,19 @@ def main() -> int:
                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_scan_tiles"),
                     "work_per_launch": work, "avg_launch_ms": k_ms}
 
+    mc = None if args.no_mc else monte_carlo(e, clusters, pulsar)
     e.close()
+    shard = None if args.no_two_stage else two_stage(clusters)
     serving = None
-    if args.serve_requests > 0:
-        from phygas_amd import serve
-        mix = serve.request_mix(args.serve_requests, args.serve_rate, seed=5, n_clusters=len(clusters))
-        serving = {"rate_hz": args.serve_rate, "requests": args.serve_requests,
-                   "mix": "70% screening chunks of 256..20,000 stars (log-uniform), 30% pulsar jobs of 100..600 "
-                          "clusters x 1,000 clones; Poisson arrivals, seed 5"}
-        for name, f in (("fast", hip.FAST), ("plain", hip.PLAIN)):
-            if f == hip.FAST and form != hip.FAST:
-                continue
-            with serve.Server(clusters, form=f, screen_slots=2, pair_slots=1, p_min=p_min) as srv:
-                serving[name] = serve.run_mix(srv, mix)
+    if not args.no_serve and args.serve_requests > 0:
+        rates_f = [float(x) for x in args.serve_rates_fast.split(",") if x]
+        rates_p = [float(x) for x in args.serve_rates_plain.split(",") if x]
+        serving = serving_sweep(clusters, p_min, rates_f if form == hip.FAST else [], rates_p, args.serve_requests)
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(clusters, stars, threads)
+        cpu = cpu_baseline(clusters, stars, pulsar, threads)
 
     value = pairs / (ms / 1000.0)
     line = {
@@ -252,6 +406,10 @@ def main() -> int:
     if plain:
         line["plain"] = plain
         line["speedup_vs_plain"] = plain["ms"] / ms
+    if mc:
+        line["monte_carlo"] = mc
+    if shard:
+        line["two_stage"] = shard
     if serving:
         line["serving"] = serving
     print(json.dumps(line, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)

@@ -752,9 +752,8 @@ static int pair_run_launches(PhxCtx *c, const PairLayout &L, const PairPlan &pla
                              const PhCluster *d_c) {
     stage_begin(c);
     if (!c->fw.use_graphs || c->timing) {
-        if (pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st)) return 1;
-        stage_mark(c, "pairs");
-        return 0;
+        auto mark = [](void *ctx, const char *name) { stage_mark((PhxCtx *)ctx, name); };
+        return pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st, c->timing ? (PairMark)mark : nullptr, c);
     }
     uint64_t key = FNV_BASIS;
     auto mix = [&](uint64_t v) { key = (key ^ v) * FNV_PRIME; };
@@ -765,7 +764,7 @@ static int pair_run_launches(PhxCtx *c, const PairLayout &L, const PairPlan &pla
         c->pgexec = nullptr;
         hipGraph_t g = nullptr;
         PHX_CK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        const int rc = pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st);
+        const int rc = pair_launch_device(L, plan, form, d_t, d_c, c->P, c->st, nullptr, nullptr);
         const hipError_t ec = hipStreamEndCapture(c->st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         PHX_CK(ec);

@@ -52,9 +52,13 @@ void pair_plan(const PhTarget *t, const PhCluster *c, int npairs, const int *ti,
                double dt_myr, PairPlan &plan);
 int pair_upload_plan(const PairLayout &L, const PairPlan &plan, const int *ti, const int *ci, hipStream_t st);
 // Draws (device), steps, traces and reduction of a planned batch from device records d_t / d_c (indexed by
-// L.ti / L.ci); results in L.out in the caller's pair order.  Launches only: capturable into a hipGraph.
+// L.ti / L.ci); results in L.out in the caller's pair order.  Launches only: capturable into a hipGraph.  mark
+// (nullable) is called after the draws, the traces and the reduction (stage timing).
+typedef void (*PairMark)(void *ctx, const char *name);
 int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, const PhTarget *d_t, const PhCluster *d_c,
-                       const Pot &P, hipStream_t st);
+                       const Pot &P, hipStream_t st, PairMark mark, void *mark_ctx);
+int pair_launch_draws(const PairLayout &L, const PairPlan &plan, const PhTarget *d_t, const PhCluster *d_c,
+                      const Pot &P, hipStream_t st);
 // Traces of n clones (persistent, queue counter d_next zeroed by a kernel first).  form: PHX_FORM_FAST = FP32
 // force, otherwise FP64.
 int launch_clones(int form, const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P,

@@ -458,8 +458,8 @@ int pair_upload_plan(const PairLayout &L, const PairPlan &plan, const int *ti, c
     return 0;
 }
 
-int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, const PhTarget *d_t, const PhCluster *d_c,
-                       const Pot &P, hipStream_t st) {
+int pair_launch_draws(const PairLayout &L, const PairPlan &plan, const PhTarget *d_t, const PhCluster *d_c,
+                      const Pot &P, hipStream_t st) {
     const int np = plan.npairs;
     if (np <= 0) return 0;
     hipLaunchKernelGGL(k_pair_draw, dim3(np), dim3(64), 0, st, d_t, d_c, L.ti, L.ci, L.seed, L.off, plan.n_clones, P,
@@ -467,8 +467,20 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
     hipLaunchKernelGGL(k_pair_steps, dim3(np), dim3(256), 0, st, L.off, L.m, L.psteps, L.pdt, plan.n_clones, L.steps,
                        L.dts);
     PHX_CK(hipGetLastError());
+    return 0;
+}
+
+int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, const PhTarget *d_t, const PhCluster *d_c,
+                       const Pot &P, hipStream_t st, PairMark mark, void *mark_ctx) {
+    const int np = plan.npairs;
+    if (np <= 0) return 0;
+    if (pair_launch_draws(L, plan, d_t, d_c, P, st)) return 1;
+    if (mark) mark(mark_ctx, "draws");
     if (launch_clones(form, L.ps, L.steps, L.dts, L.res, P, plan.slots, L.next, st)) return 1;
-    return launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, np, plan.n_clones, L.out, st);
+    if (mark) mark(mark_ctx, "clones");
+    if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, np, plan.n_clones, L.out, st)) return 1;
+    if (mark) mark(mark_ctx, "reduce");
+    return 0;
 }
 
 }  // namespace phx
@@ -570,7 +582,7 @@ extern "C" int phx_pair_noalloc(int form, const PhTarget *t, const PhCluster *c,
     plan.npairs = npairs;
     plan.n_clones = n_clones;
     plan.slots = slots;
-    if (pair_launch_device(L, plan, form, L.tg, L.cl, P, st)) return 1;
+    if (pair_launch_device(L, plan, form, L.tg, L.cl, P, st, nullptr, nullptr)) return 1;
     PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, st));
     PHX_CK(hipStreamSynchronize(st));
     return 0;

@@ -12,8 +12,9 @@
 // depends only on its own inputs and per-pair seeds, so a request's rows are the same alone or in any mix.
 //
 // Admission.  A request's footprint is estimated from its size (screening: tracks, tiles and candidate
-// buffers per star; pairs: 300 bytes per clone); a batch is started only while the footprints of the
-// batches in flight stay under the budget.  A single request larger than the budget runs alone.
+// buffers per star; pairs: 300 bytes per clone slot, about twice the Monte Carlo layout's 132); a batch is
+// started only while the footprints of the batches in flight stay under the budget.  A single request larger
+// than the budget runs alone.
 #include <math.h>
 #include <string.h>
 
@@ -286,7 +287,7 @@ extern "C" long phx_server_submit_screen(PhxServer *s, const PhTarget *stars, in
 
 extern "C" long phx_server_submit_pair(PhxServer *s, const PhTarget *target, const int *cluster_idx, int n,
                                        int n_clones, double dt_myr) {
-    if (n <= 0 || n_clones <= 0 || n_clones > 4096) { set_error("phx_server_submit_pair", "bad request size"); return -2; }
+    if (n <= 0 || n_clones <= 0) { set_error("phx_server_submit_pair", "bad request size"); return -2; }
     if (s->cfg.pair_slots <= 0) { set_error("phx_server_submit_pair", "the server has no pair slot"); return -2; }
     for (int i = 0; i < n; i++)
         if (cluster_idx[i] < 0 || cluster_idx[i] >= (int)s->clusters.size()) {

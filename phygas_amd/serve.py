@@ -164,7 +164,7 @@ def run_mix(server: Server, mix, n_clones: int = 1000, dt_myr: float = 0.05) -> 
     for rid in ids:
         server.wait(rid)
     t1 = time.monotonic()
-    lat, lat_s, lat_p, pairs, clones = [], [], [], 0, 0
+    lat, lat_s, lat_p, lat_big, pairs, clones = [], [], [], [], 0, 0
     first = last = None
     for (ta, kind, payload), rid in zip(mix, ids):
         inf = server.info(rid)
@@ -173,6 +173,8 @@ def run_mix(server: Server, mix, n_clones: int = 1000, dt_myr: float = 0.05) -> 
         last = inf["done"] if last is None else max(last, inf["done"])
         if kind == "screen":
             lat_s.append(inf["latency"])
+            if len(payload) >= 10000:
+                lat_big.append(inf["latency"])
             pairs += len(payload) * len(server.clusters)
         else:
             lat_p.append(inf["latency"])
@@ -187,5 +189,7 @@ def run_mix(server: Server, mix, n_clones: int = 1000, dt_myr: float = 0.05) -> 
             "p50_s": pct(lat, 50), "p95_s": pct(lat, 95), "p99_s": pct(lat, 99),
             "screen_p50_s": pct(lat_s, 50), "screen_p99_s": pct(lat_s, 99),
             "pair_p50_s": pct(lat_p, 50), "pair_p99_s": pct(lat_p, 99),
+            "large_screen_requests": len(lat_big), "large_screen_p50_s": pct(lat_big, 50),
+            "large_screen_p99_s": pct(lat_big, 99),
             "span_s": span, "pairs_per_s": pairs / span if span > 0 else None,
             "clones_per_s": clones / span if span > 0 else None, "wall_s": t1 - t0}

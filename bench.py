@@ -251,6 +251,16 @@ def two_stage(clusters, n_stars: int = 100000, n_clones: int = 1000, p_stage2: f
             out[f"stage2_{name}_ms"] = ms
             out[f"stage2_{name}_clone_steps_per_s"] = steps / (ms / 1e3)
         cap, rep = e.pair_graph_stats()
+        # stage 2 at the config's expected scale (~60,000 pairs): the real rows with the highest p_r50
+        big = rows[rows["flags"] == 0]
+        big = big[np.argsort(-big["p_r50"], kind="stable")][:60000]
+        bsi, bci = big["star"].astype(np.int32), big["cluster"].astype(np.int32)
+        bsteps = float(np.sum(np.ceil(clusters["tmax_myr"][bci] / 0.05))) * n_clones
+        e.pair_loaded(bsi, bci, n_clones, 0.05, hip.FAST)
+        bms = _median_ms(lambda: e.pair_loaded(bsi, bci, n_clones, 0.05, hip.FAST), 2)
+        out["stage2_at_60k_pairs"] = {"pairs": int(len(bsi)), "p_r50_min": float(big["p_r50"].min()) if len(big) else None,
+                                      "clones": int(len(bsi)) * n_clones, "clone_steps": bsteps, "fast_ms": bms,
+                                      "fast_clone_steps_per_s": bsteps / (bms / 1e3)}
     out.update({"stage1_ms": s1, "stage1_pairs": 2 * n_stars * len(clusters),
                 "stage1_pairs_per_s": 2 * n_stars * len(clusters) / (s1 / 1e3), "rows": int(len(rows)),
                 "stage2_pairs": int(len(si)), "stage2_clone_steps": steps,
@@ -295,8 +305,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rates-fast", default="15,60,240,960")
     ap.add_argument("--serve-rates-plain", default="15,30,60")
-    ap.add_argument("--tune", action="store_true", help="run the autotuner before timing (the library's default "
-                    "grids are its choices on this unit)")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner (time tile of the scan's first level, "
+                    "stars per task, launch grids) before timing; the library's defaults are its choices on this unit")
     ap.add_argument("--split", type=int, default=0, help="sub-passes of a fast pass on their own streams "
                     "(0: the library's default, 3)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
@@ -331,7 +341,7 @@ def main() -> int:
     form = hip.FAST if args.form == "fast" else hip.PLAIN
     tune = None
     if form == hip.FAST and args.tune:
-        tune = e.autotune(hip.FAST, chi2_max, 64, p_min)   # launch grids, chosen before the timed steps
+        tune = e.autotune_ex(hip.FAST, chi2_max, 64, p_min)   # every setting, chosen before the timed steps
     ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
     rows = e.rows()
     kept = e.kept()

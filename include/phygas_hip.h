@@ -121,6 +121,22 @@ int phx_autotune(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_mi
  * apply an earlier phx_autotune result without timing again.  No counterpart in the reference. */
 int phx_set_grids(PhxCtx *ctx, const int *grids);
 
+/* Every tunable setting of the fast pass.  stars_per_task: stars of one scan task (8 or 16; each task is one
+ * wavefront over 64 clusters of similar track length, the clusters sorted by length into buckets of 64);
+ * coarse_tiles: the scan's first level on 16-point time tiles, then their 8-point tiles (1, default), or on
+ * the 8-point tiles directly (0).  Rows do not depend on any setting.  No counterpart in the reference. */
+typedef struct {
+    int scan_blocks, items_blocks, reduce_blocks, finish_blocks;   /* launch grids; <= 0: the library's default */
+    int stars_per_task;
+    int coarse_tiles;
+    double pass_ms;   /* phx_autotune_ex: the pass time with the settings chosen */
+} PhxTuning;
+/* phx_autotune over every setting (the time tile of the scan's first level, stars per task, the four grids);
+ * the choices stay set on the context and are returned in *out. */
+int phx_autotune_ex(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min, PhxTuning *out);
+int phx_set_tuning(PhxCtx *ctx, const PhxTuning *t);
+int phx_get_tuning(PhxCtx *ctx, PhxTuning *t);
+
 /* Parity access to intermediate device results of the last plain pass / the cluster tables. */
 int phx_cluster_tracks(PhxCtx *ctx, int *n, double *pos, double *vel, double *S, int npts_cap);
 int phx_star_tracks(PhxCtx *ctx, int first, int count, double *trk, int npts_cap);
@@ -161,6 +177,14 @@ int phx_pair_loaded(PhxCtx *ctx, int form, int npairs, const int *star, const in
                     double dt_myr, PhResult *res);
 /* Captures and replays of the device-draw Monte Carlo graphs. */
 int phx_pair_graph_stats(PhxCtx *ctx, long *captures, long *replays);
+/* Bucket width of the persistent clone kernel: 64-clone chunks per queue grab (1..64; default 1).  Wider buckets
+ * take fewer queue atomics, narrower ones balance the tail better; results do not depend on it.  No
+ * counterpart in the reference. */
+int phx_set_pair_bucket(PhxCtx *ctx, int chunks);
+/* The bucket width chosen by timing phx_pair_loaded on the caller's batch over 1, 2, 4 and 8 chunks; it stays set
+ * on the context.  *bucket, *ms: the choice and its time.  No counterpart in the reference. */
+int phx_autotune_pairs(PhxCtx *ctx, int form, int npairs, const int *star, const int *cluster, int n_clones,
+                       double dt_myr, int *bucket, double *ms);
 
 /* ---------------------------------------------------------------- 3. serving loop
  * Requests (screening chunks, Monte Carlo pair jobs) submitted as they arrive and batched continuously onto

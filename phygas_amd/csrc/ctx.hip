@@ -92,6 +92,7 @@ struct PhxCtx {
     hipGraphExec_t pgexec = nullptr;
     uint64_t pgkey = 0;
     long pcaptures = 0, preplays = 0;
+    int pair_bucket = 1;   // 64-clone chunks per queue grab of the clone kernel (phx_set_pair_bucket)
 };
 
 static FastWork &part_fw(PhxCtx *c, int i) { return i == 0 ? c->fw : c->pfw[i]; }
@@ -422,6 +423,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         if (i > 0) {   // the launch settings of part 0
             W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;
             W.reduce_blocks = c->fw.reduce_blocks; W.finish_blocks = c->fw.finish_blocks;
+            W.scan_spw = c->fw.scan_spw; W.scan_coarse = c->fw.scan_coarse;
             W.use_graphs = c->fw.use_graphs;
         }
         if (fast_screen(c->fast, W, ai, part_st(c, i), nullptr, nullptr)) return 1;
@@ -494,31 +496,34 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
     return last_counts(c, n, &r);
 }
 
-// Launch grids of the fast pass chosen by timing each stage on the loaded batch: for the tile scan, the item
-// evaluation, the kept-pair reduction and the finish, every grid in a short list is tried (the others held
-// at their current values) and the fastest kept.  Results do not depend on the grids.
-extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms) {
+// Launch settings of the fast pass chosen by timing whole passes on the loaded batch as the library runs them
+// (sub-passes overlapping), one knob after another with the others held, best of two passes per option after
+// one not counted: the scan's first level (16-point coarse tiles or the 8-point tiles), its stars per task, and
+// the launch grids of the scan, the item evaluation, the kept-pair reduction and the finish.  Rows do not
+// depend on any of them.  knob_ms[k]: the best pass time while knob k was tuned.
+static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, double knob_ms[6]) {
     PHX_REQUIRE(form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "the autotuner tunes the fast forms");
     PHX_REQUIRE(!c->st_h.empty(), "load stars first");
     if (phx_screen(c, form, chi2_max, n_samp, p_min)) return 1;
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;   // sizes the buffers
-    // each option is timed on whole passes as the library runs them (sub-passes overlapping), best of two
     struct Knob { int *v; int opts[5]; };
-    Knob knobs[4] = {{&c->fw.scan_blocks, {512, 576, 768, 1024, 2048}},
+    Knob knobs[6] = {{&c->fw.scan_coarse, {1, 0, -1, -1, -1}},
+                     {&c->fw.scan_spw, {16, 8, -1, -1, -1}},
+                     {&c->fw.scan_blocks, {512, 576, 768, 1024, 2048}},
                      {&c->fw.items_blocks, {4096, 8192, 16384, 32768, 65536}},
-                     {&c->fw.reduce_blocks, {1024, 2048, 4096, 8192, 0}},
-                     {&c->fw.finish_blocks, {1024, 2048, 4096, 8192, 0}}};
+                     {&c->fw.reduce_blocks, {1024, 2048, 4096, 8192, -1}},
+                     {&c->fw.finish_blocks, {1024, 2048, 4096, 8192, -1}}};
     const bool timing = c->timing;
     c->timing = false;
     int rc = 0;
-    for (int q = 0; q < 4 && rc == 0; q++) {
+    for (int q = 0; q < 6 && rc == 0; q++) {
         int best = *knobs[q].v;
         double best_ms = 1e300;
-        for (int o = 0; o < 5 && knobs[q].opts[o] > 0; o++) {
+        for (int o = 0; o < 5 && knobs[q].opts[o] >= 0; o++) {
             *knobs[q].v = knobs[q].opts[o];
             double t = 1e300;
-            for (int rep = 0; rep < 3 && rc == 0; rep++) {   // the first pass with new grids is not counted
+            for (int rep = 0; rep < 3 && rc == 0; rep++) {   // the first pass with new settings is not counted
                 const auto t0 = std::chrono::steady_clock::now();
                 if ((rc = phx_screen(c, form, chi2_max, n_samp, p_min)) != 0) break;
                 if (hipStreamSynchronize(c->st) != hipSuccess) { rc = 1; break; }
@@ -528,11 +533,21 @@ extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, do
             if (rc == 0 && t < best_ms) { best_ms = t; best = knobs[q].opts[o]; }
         }
         *knobs[q].v = best;
-        if (grids) grids[q] = best;
-        if (ms) ms[q] = best_ms;
+        knob_ms[q] = best_ms;
     }
     c->timing = timing;
     return rc;
+}
+
+extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms) {
+    double kms[6];
+    if (int rc = autotune_fast(c, form, chi2_max, n_samp, p_min, kms)) return rc;
+    const int g[4] = {c->fw.scan_blocks, c->fw.items_blocks, c->fw.reduce_blocks, c->fw.finish_blocks};
+    for (int i = 0; i < 4; i++) {
+        if (grids) grids[i] = g[i];
+        if (ms) ms[i] = kms[2 + i];
+    }
+    return 0;
 }
 
 extern "C" int phx_set_grids(PhxCtx *c, const int *grids) {
@@ -542,6 +557,38 @@ extern "C" int phx_set_grids(PhxCtx *c, const int *grids) {
     c->fw.items_blocks = grids[1];
     c->fw.reduce_blocks = grids[2];
     c->fw.finish_blocks = grids[3];
+    return 0;
+}
+
+extern "C" int phx_get_tuning(PhxCtx *c, PhxTuning *t) {
+    t->scan_blocks = c->fw.scan_blocks;
+    t->items_blocks = c->fw.items_blocks;
+    t->reduce_blocks = c->fw.reduce_blocks;
+    t->finish_blocks = c->fw.finish_blocks;
+    t->stars_per_task = c->fw.scan_spw;
+    t->coarse_tiles = c->fw.scan_coarse;
+    t->pass_ms = 0;
+    return 0;
+}
+
+extern "C" int phx_autotune_ex(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, PhxTuning *out) {
+    double kms[6];
+    if (int rc = autotune_fast(c, form, chi2_max, n_samp, p_min, kms)) return rc;
+    if (out) {
+        phx_get_tuning(c, out);
+        out->pass_ms = kms[5];
+    }
+    return 0;
+}
+
+extern "C" int phx_set_tuning(PhxCtx *c, const PhxTuning *t) {
+    PHX_REQUIRE(t != nullptr, "tuning is null");
+    PHX_REQUIRE(t->stars_per_task == 8 || t->stars_per_task == 16, "stars per task must be 8 or 16");
+    PHX_REQUIRE(t->coarse_tiles == 0 || t->coarse_tiles == 1, "coarse_tiles must be 0 or 1");
+    const int g[4] = {t->scan_blocks, t->items_blocks, t->reduce_blocks, t->finish_blocks};
+    if (int rc = phx_set_grids(c, g)) return rc;
+    c->fw.scan_spw = t->stars_per_task;
+    c->fw.scan_coarse = t->coarse_tiles;
     return 0;
 }
 
@@ -733,7 +780,7 @@ extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int 
     PHX_CK(hipMemcpyAsync(L.steps, steps, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(L.dts, dts, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
     stage_begin(c);
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, n, L.next, c->st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, n, c->pair_bucket, L.next, c->st)) return 1;
     stage_mark(c, "clones");
     std::vector<double> r(3 * (size_t)n);
     PHX_CK(hipMemcpyAsync(r.data(), L.res, 24 * (size_t)n, hipMemcpyDeviceToHost, c->st));
@@ -758,6 +805,7 @@ static int pair_run_launches(PhxCtx *c, const PairLayout &L, const PairPlan &pla
     uint64_t key = FNV_BASIS;
     auto mix = [&](uint64_t v) { key = (key ^ v) * FNV_PRIME; };
     mix((uint64_t)plan.npairs); mix((uint64_t)plan.n_clones); mix((uint64_t)plan.slots); mix((uint64_t)form);
+    mix((uint64_t)plan.bucket);
     mix((uint64_t)(uintptr_t)L.ps); mix((uint64_t)(uintptr_t)d_t); mix((uint64_t)(uintptr_t)d_c);
     if (!c->pgexec || c->pgkey != key) {
         if (c->pgexec) (void)hipGraphExecDestroy(c->pgexec);
@@ -791,6 +839,7 @@ static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhClu
     }
     PairPlan plan;
     pair_plan(t, cl, npairs, ti, ci, n_clones, dt_myr, plan);
+    plan.bucket = c->pair_bucket;
     PairLayout L;
     if (pair_buffers(c, plan.slots, npairs, nt, ncl, L)) return 1;
     PHX_CK(hipMemcpyAsync(L.tg, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
@@ -823,12 +872,45 @@ extern "C" int phx_pair_loaded(PhxCtx *c, int form, int npairs, const int *star,
     if (upload_stars(c)) return 1;
     PairPlan plan;
     pair_plan(c->st_h.data(), c->cl_h.data(), npairs, star, cluster, n_clones, dt_myr, plan);
+    plan.bucket = c->pair_bucket;
     PairLayout L;
     if (pair_buffers(c, plan.slots, npairs, 0, 0, L)) return 1;
     if (pair_upload_plan(L, plan, star, cluster, c->st)) return 1;
     if (pair_run_launches(c, L, plan, form, c->sr.as<PhTarget>(), c->cl.as<PhCluster>())) return 1;
     PHX_CK(hipMemcpyAsync(res, L.out, sizeof(PhResult) * (size_t)npairs, hipMemcpyDeviceToHost, c->st));
     PHX_CK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+extern "C" int phx_set_pair_bucket(PhxCtx *c, int chunks) {
+    PHX_REQUIRE(chunks >= 1 && chunks <= 64, "bucket width must be 1..64 chunks of 64 clones");
+    c->pair_bucket = chunks;
+    return 0;
+}
+
+// The clone kernel's bucket width chosen by timing phx_pair_loaded on the caller's batch (results do not depend
+// on it); the choice stays set.  No counterpart in the reference.
+extern "C" int phx_autotune_pairs(PhxCtx *c, int form, int npairs, const int *star, const int *cluster, int n_clones,
+                                  double dt_myr, int *bucket, double *ms) {
+    PHX_REQUIRE(npairs > 0, "no pairs to tune on");
+    std::vector<PhResult> res(npairs);
+    const int opts[4] = {1, 2, 4, 8};
+    int best = c->pair_bucket;
+    double best_ms = 1e300;
+    for (int o : opts) {
+        c->pair_bucket = o;
+        double t = 1e300;
+        for (int rep = 0; rep < 3; rep++) {   // the first call with a new width is not counted
+            const auto t0 = std::chrono::steady_clock::now();
+            if (int rc = phx_pair_loaded(c, form, npairs, star, cluster, n_clones, dt_myr, res.data())) return rc;
+            const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rep > 0) t = std::min(t, dt);
+        }
+        if (t < best_ms) { best_ms = t; best = o; }
+    }
+    c->pair_bucket = best;
+    if (bucket) *bucket = best;
+    if (ms) *ms = best_ms;
     return 0;
 }
 
@@ -868,7 +950,7 @@ extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster 
     PHX_CK(hipMemcpyAsync(L.dts, dts.data(), 8 * n, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(L.m, m.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
     stage_begin(c);
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, plan.slots, L.next, c->st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, plan.slots, c->pair_bucket, L.next, c->st)) return 1;
     stage_mark(c, "clones");
     if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, npairs, n_clones, L.out, c->st)) return 1;
     stage_mark(c, "reduce");

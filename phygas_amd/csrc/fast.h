@@ -24,6 +24,9 @@ struct FastWork {
     // launch grids (0: the defaults, chosen on the 40,000-star x 4,087-cluster unit run as three sub-passes);
     // phx_autotune picks them by timing whole passes on the loaded batch
     int scan_blocks = 0, items_blocks = 0, reduce_blocks = 0, finish_blocks = 0;
+    // the scan's stars per task (8 or 16) and its first level (1: 16-point coarse tiles, then their 8-point
+    // tiles; 0: the 8-point tiles directly)
+    int scan_spw = 16, scan_coarse = 1;
     // hipGraph of the pass's launches, replayed while sizes, parameters and buffers stay the same
     bool use_graphs = false;
     void *graph = nullptr, *gexec = nullptr;

@@ -360,10 +360,15 @@ __device__ __forceinline__ TileRecF lds_get_recf(const float (*d)[64], int i) {
 }
 
 constexpr int L1G = 2;   // star records of the level-1 loop held at once (scalar registers)
-template <typename MaskT>
+// SPWT stars per task (8 or 16; the star records are padded to SPW = 16).  Coarse: the first level on 16-point
+// coarse tiles, then the fine tiles of the coarse tiles left (as above); otherwise both levels on the 8-point
+// tiles directly (more level-1 tests, no second queue).  The autotuner picks both; rows do not depend on them.
+template <typename MaskT, int SPWT, bool Coarse>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_scan_tiles(const TileRec1 *__restrict__ crec1c,
                                                     const TileRecF *__restrict__ crecfc,
                                                     const TileRecF *__restrict__ crecf,
+                                                    const TileRec1 *__restrict__ crec1,
+                                                    const TileRec1 *__restrict__ srec1,
                                                     const int *__restrict__ ntc_sorted,
                                                     const int *__restrict__ cn_sorted, const int *__restrict__ perm,
                                                     int nc, int nc_pad, const TileRec1 *__restrict__ srec1c,
@@ -374,7 +379,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     unsigned long long *__restrict__ next_task,
                                                     unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
-    __shared__ unsigned short q_s[4][SPW * 64];
+    static_assert(SPWT == 8 || SPWT == 16, "8 or 16 stars per task");
+    __shared__ unsigned short q_s[4][SPWT * 64];
     __shared__ float c2_s[4][CFF][64];
     __shared__ unsigned l2_s[4][64];
     __shared__ int nt_s[4][64];
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const float chi2cf = up(chi2c);   // the fine level's FP32 cut, rounded up
     unsigned short *q = q_s[w];
     float (*c2s)[64] = c2_s[w];
-    const int ncb = (nc + 63) / 64, nsb = (ns + SPW - 1) / SPW;
+    const int ncb = (nc + 63) / 64, nsb = (ns + SPWT - 1) / SPWT;
     const int ntasks = ncb * nsb;
     // exclusive prefix of a per-lane count over the wavefront; the total into *tot
     auto excl_prefix = [&](unsigned cnt, int *tot) {
@@ -413,22 +419,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int nt = cs < nc ? ntc_sorted[cs] : 0;
         const int ncl = cs < nc ? cn_sorted[cs] : 0;
         const int ntmax = __builtin_amdgcn_readfirstlane(nt);   // sorted descending: lane 0 has the longest
-        const int s0 = sb * SPW;
-        const int nsv = min(SPW, ns - s0);   // stars of the block (the rest is padding)
+        const int s0 = sb * SPWT;
+        const int nsv = min(SPWT, ns - s0);   // stars of the block (the rest is padding)
         nt_s[w][lane] = nt;
         unsigned n_l1 = 0, n_l2 = 0;   // tests of this task (wave-uniform)
         unsigned fine_l2 = 0;          // fine level-2 tests of this lane's queue entries
-        MaskT mask[SPW];
+        MaskT mask[SPWT];
 #pragma unroll
-        for (int i = 0; i < SPW; i++) mask[i] = 0;
-        for (int J = 0; 2 * J < ntmax; J++) {
+        for (int i = 0; i < SPWT; i++) mask[i] = 0;
+        if constexpr (!Coarse) {
+            // one level of 8-point tiles: level 1, then the certificate on the survivors, per tile j
+            for (int j = 0; j < ntmax; j++) {
+                const bool cv = j < nt;
+                const TileRec1 c1 = crec1[(size_t)j * nc_pad + cs];
+                const float t1 = cv ? tile_t1<TILE>(ncl, j) : 0.0f;
+                n_l1 += (unsigned)nsv * (unsigned)__popcll(__ballot(cv));
+                unsigned l1 = 0u;
+#pragma unroll
+                for (int i0 = 0; i0 < SPWT; i0 += L1G) {
+                    TileRec1 sr[L1G];
+#pragma unroll
+                    for (int u = 0; u < L1G; u++) sr[u] = srec1[(size_t)(s0 + i0 + u) * ntl + j];
+#pragma unroll
+                    for (int u = 0; u < L1G; u++)
+                        if (l1_pass<TILE>(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
+                }
+                if (!cv) l1 = 0u;
+                int nq = 0;
+                const unsigned o1 = excl_prefix((unsigned)__popc(l1), &nq);
+                if (nq == 0) continue;
+                n_l2 += nq;
+                enqueue(l1, o1);
+                if (l1) lds_put_recf(c2s, lane, crecf[(size_t)j * nc_pad + cs]);
+                l2_s[w][lane] = 0u;
+                wave_sync();
+                for (int e0 = 0; e0 < nq; e0 += 64) {   // wave-uniform
+                    const int e = e0 + lane;
+                    if (e < nq) {
+                        const int ent = q[e], i = ent >> 6, c = ent & 63;
+                        if (l2f_pass(srecf[(size_t)(s0 + i) * ntl + j], lds_get_recf(c2s, c), chi2cf))
+                            atomicOr(&l2_s[w][c], 1u << i);
+                    }
+                }
+                wave_sync();
+                const unsigned l2 = l2_s[w][lane];
+#pragma unroll
+                for (int i = 0; i < SPWT; i++) mask[i] |= (MaskT)((l2 >> i) & 1u) << j;
+                wave_sync();   // q / c2s / l2_s are rewritten by the next tile
+            }
+        }
+        for (int J = 0; Coarse && 2 * J < ntmax; J++) {
             const bool cv = 2 * J < nt;
             const TileRec1 c1 = crec1c[(size_t)J * nc_pad + cs];
             const float t1 = cv ? tile_t1<TILE2>(ncl, J) : 0.0f;
             n_l1 += (unsigned)nsv * (unsigned)__popcll(__ballot(cv));
             unsigned l1 = 0u;
 #pragma unroll
-            for (int i0 = 0; i0 < SPW; i0 += L1G) {   // star records of the padded block: wave-uniform loads
+            for (int i0 = 0; i0 < SPWT; i0 += L1G) {   // star records of the padded block: wave-uniform loads
                 TileRec1 sr[L1G];
 #pragma unroll
                 for (int u = 0; u < L1G; u++) sr[u] = srec1c[(size_t)(s0 + i0 + u) * ntlc + J];
@@ -483,7 +530,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             wave_sync();
             const unsigned l2 = l2_s[w][lane];
 #pragma unroll
-            for (int i = 0; i < SPW; i++)
+            for (int i = 0; i < SPWT; i++)
                 mask[i] |= (MaskT)((l2 >> (2 * i)) & 3u) << (2 * J);
             wave_sync();   // q / c2s / l2_s are rewritten by the next coarse tile
         }
@@ -497,7 +544,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // the task's candidates take one range of the list (one atomic per task)
         unsigned total = 0;
 #pragma unroll
-        for (int i = 0; i < SPW; i++) total += (unsigned)__popcll(__ballot(cs < nc && s0 + i < ns && mask[i] != 0));
+        for (int i = 0; i < SPWT; i++) total += (unsigned)__popcll(__ballot(cs < nc && s0 + i < ns && mask[i] != 0));
         if (total == 0) continue;
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(ncand, (unsigned long long)total);
@@ -505,7 +552,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const unsigned long long below = (1ull << lane) - 1ull;
         const int cl = cs < nc ? perm[cs] : 0;
 #pragma unroll
-        for (int i = 0; i < SPW; i++) {
+        for (int i = 0; i < SPWT; i++) {
             const bool hit = cs < nc && s0 + i < ns && mask[i] != 0;
             const unsigned long long b = __ballot(hit);
             if (hit) {
@@ -1357,14 +1404,20 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tiles");
     const double chi2c = a.chi2_max * (1.0 + CUT_REL);
-    auto scan = ntl <= 32 ? k_scan_tiles<unsigned> : k_scan_tiles<unsigned long long>;
+    const bool wide = ntl > 32, coarse = W.scan_coarse != 0;
+    const int spw = W.scan_spw == 8 ? 8 : 16;
+    auto scan = wide ? (coarse ? (spw == 8 ? k_scan_tiles<unsigned long long, 8, true> : k_scan_tiles<unsigned long long, 16, true>)
+                               : (spw == 8 ? k_scan_tiles<unsigned long long, 8, false> : k_scan_tiles<unsigned long long, 16, false>))
+                     : (coarse ? (spw == 8 ? k_scan_tiles<unsigned, 8, true> : k_scan_tiles<unsigned, 16, true>)
+                               : (spw == 8 ? k_scan_tiles<unsigned, 8, false> : k_scan_tiles<unsigned, 16, false>));
     // default grid: 2 blocks (2 wavefronts per SIMD) per CU, below the 3 that fit (VGPRs): the scan's tasks
     // are taken dynamically, so a grid the device holds at once leaves no blocks waiting, and the room left
     // on each SIMD lets the other sub-pass's kernels run beside the scan instead of queueing behind it
     const int scan_grid = W.scan_blocks > 0 ? W.scan_blocks : T.n_cu * 2;
     hipLaunchKernelGGL(scan, dim3(scan_grid), dim3(256), 0, st,
                        (const TileRec1 *)T.mem[T_CREC1C], (const TileRecF *)T.mem[T_CRECFC],
-                       (const TileRecF *)T.mem[T_CRECF], (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
+                       (const TileRecF *)T.mem[T_CRECF], (const TileRec1 *)T.mem[T_CREC1], (const TileRec1 *)srec1,
+                       (const int *)T.mem[T_CNT], (const int *)T.mem[T_CNS],
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
                        (const TileRecF *)W.mem[W_SRECFC], (const TileRecF *)W.mem[W_SRECF], ns, ntl, n_coarse(ntl),
                        up(sqrt(chi2c)), chi2c,
@@ -1436,7 +1489,8 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };
     mix((uint64_t)a.ns); mix((uint64_t)a.nc); mix((uint64_t)a.npts); mix((uint64_t)a.n_samp);
     mix((uint64_t)a.exact_tracks); mix((uint64_t)a.track_lanes); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
-    mix((uint64_t)W.reduce_blocks); mix((uint64_t)W.finish_blocks);
+    mix((uint64_t)W.reduce_blocks); mix((uint64_t)W.finish_blocks); mix((uint64_t)W.scan_spw);
+    mix((uint64_t)W.scan_coarse);
     double dv[2] = {a.chi2_max, a.p_min};
     uint64_t bits[2];
     memcpy(bits, dv, sizeof bits);

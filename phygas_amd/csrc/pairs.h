@@ -42,6 +42,7 @@ PairLayout pair_layout(char *base, long slots, int npairs, int ntg, int ncl);
 // radii, and slot offsets with the pairs ranked longest trace-back first (n_clones slots each).
 struct PairPlan {
     int npairs = 0, n_clones = 0, max_steps = 0;
+    int bucket = 1;   // 64-clone chunks per queue grab of the clone kernel
     long slots = 0;
     std::vector<uint64_t> seed;
     std::vector<int> psteps;
@@ -59,10 +60,10 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
                        const Pot &P, hipStream_t st, PairMark mark, void *mark_ctx);
 int pair_launch_draws(const PairLayout &L, const PairPlan &plan, const PhTarget *d_t, const PhCluster *d_c,
                       const Pot &P, hipStream_t st);
-// Traces of n clones (persistent, queue counter d_next zeroed by a kernel first).  form: PHX_FORM_FAST = FP32
-// force, otherwise FP64.
+// Traces of n clones (persistent, queue counter d_next zeroed by a kernel first; each queue grab takes cw
+// 64-clone chunks).  form: PHX_FORM_FAST = FP32 force, otherwise FP64.
 int launch_clones(int form, const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P,
-                  long n, unsigned *d_next, hipStream_t st);
+                  long n, int cw, unsigned *d_next, hipStream_t st);
 // Quantiles and fractions of each pair's clones; max_m bounds every m[p] (chooses the LDS sort or selection).
 int launch_pair_reduce(const double *d_res, const long *d_off, const int *d_m, const double *d_r50_pc,
                        const double *d_rtot_pc, int npairs, int max_m, PhResult *d_out, hipStream_t st);

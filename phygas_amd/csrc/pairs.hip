@@ -184,21 +184,24 @@ __device__ __forceinline__ void trace_clone(const double *__restrict__ p, int ns
     R[2] = vbest;
 }
 
-// Persistent grid: each wavefront takes the next 64-clone chunk from *next (zeroed before the launch) until
-// the chunks run out; every wavefront reaches that exit.
+// Persistent grid: each wavefront takes the next bucket of cw 64-clone chunks from *next (zeroed before the
+// launch) and traces them one after another, until the buckets run out; every wavefront reaches that exit.
+// Wider buckets take fewer queue atomics, narrower ones balance the tail better (the autotuner picks cw).
 template <int Fast>
 __global__ __launch_bounds__(256) void k_clones(const double *__restrict__ ps, const int *__restrict__ steps,
                                                 const double *__restrict__ dts, double *__restrict__ res, Pot P,
-                                                PotF Q, long n, unsigned *__restrict__ next) {
+                                                PotF Q, long n, int cw, unsigned *__restrict__ next) {
     const int lane = threadIdx.x & 63;
-    const long nchunks = (n + 63) / 64;
+    const long nchunks = (n + 63) / 64, nbuckets = (nchunks + cw - 1) / cw;
     for (;;) {
-        unsigned c = 0;
-        if (lane == 0) c = atomicAdd(next, 1u);
-        c = __shfl(c, 0, 64);
-        if ((long)c >= nchunks) break;
-        const long k = (long)c * 64 + lane;
-        if (k < n) trace_clone<Fast>(ps + (size_t)k * 12, steps[k], dts[k], P, Q, res + (size_t)k * 3);
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(next, 1u);
+        b = __shfl(b, 0, 64);
+        if ((long)b >= nbuckets) break;
+        for (int u = 0; u < cw; u++) {
+            const long k = ((long)b * cw + u) * 64 + lane;
+            if (k < n) trace_clone<Fast>(ps + (size_t)k * 12, steps[k], dts[k], P, Q, res + (size_t)k * 3);
+        }
     }
 }
 
@@ -368,7 +371,7 @@ __global__ __launch_bounds__(256) void k_pair_reduce_select(const double *__rest
 }
 
 // ---------------------------------------------------------------- launches
-static int clone_grid(int fast, long n) {
+static int clone_grid(int fast, long n, int cw) {
     static int per_cu[2] = {0, 0}, cus = 0;
     if (!cus) {
         int dev = 0;
@@ -383,23 +386,24 @@ static int clone_grid(int fast, long n) {
         per_cu[0] = std::max(b0, 1);
         per_cu[1] = std::max(b1, 1);
     }
-    const long chunks = (n + 63) / 64, blocks_needed = (chunks + 3) / 4;
+    const long buckets = ((n + 63) / 64 + cw - 1) / cw, blocks_needed = (buckets + 3) / 4;
     return (int)std::min<long>((long)per_cu[fast] * cus, std::max(blocks_needed, 1L));
 }
 
 int launch_clones(int form, const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P,
-                  long n, unsigned *d_next, hipStream_t st) {
+                  long n, int cw, unsigned *d_next, hipStream_t st) {
     if (n <= 0) return 0;
+    PHX_REQUIRE(cw >= 1 && cw <= 64, "clone bucket width must be 1..64 chunks");
     const int fast = form == PHX_FORM_FAST ? 1 : 0;
-    const int grid = clone_grid(fast, n);
+    const int grid = clone_grid(fast, n, cw);
     PHX_REQUIRE(grid > 0, "cannot size the clone grid");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
     if (fast)
         hipLaunchKernelGGL(k_clones<1>, dim3(grid), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res, P, make_potf(P), n,
-                           d_next);
+                           cw, d_next);
     else
         hipLaunchKernelGGL(k_clones<0>, dim3(grid), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res, P, make_potf(P), n,
-                           d_next);
+                           cw, d_next);
     PHX_CK(hipGetLastError());
     return 0;
 }
@@ -476,7 +480,7 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
     if (np <= 0) return 0;
     if (pair_launch_draws(L, plan, d_t, d_c, P, st)) return 1;
     if (mark) mark(mark_ctx, "draws");
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, P, plan.slots, L.next, st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, P, plan.slots, plan.bucket, L.next, st)) return 1;
     if (mark) mark(mark_ctx, "clones");
     if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, np, plan.n_clones, L.out, st)) return 1;
     if (mark) mark(mark_ctx, "reduce");

@@ -80,9 +80,13 @@ def lib() -> C.CDLL:
         "phx_pair_noalloc": (_I, [_I, _P, _P, _I, _P, _P, _I, _D, _P, _P]),
         "phx_pair_noalloc_limit": (C.c_size_t, []),
         "phx_pair_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
+        "phx_set_pair_bucket": (_I, [_P, _I]),
+        "phx_autotune_pairs": (_I, [_P, _I, _I, _P, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
         "phx_set_grids": (_I, [_P, _P]),
+        "phx_autotune_ex": (_I, [_P, _I, _D, _I, _D, _P]),
+        "phx_set_tuning": (_I, [_P, _P]), "phx_get_tuning": (_I, [_P, _P]),
         "phx_prefilter": (_I, [_P, _P, _I, _P, _I, _P]),
     }
     for name, (res, args) in sig.items():
@@ -147,6 +151,16 @@ def crmath(fn: str, x, y=None, device: bool = False) -> np.ndarray:
     out = np.zeros_like(x)
     _check(lib().phx_check_crmath(CRMATH_FUNCS[fn], int(device), _p(x), _p(y), x.size, _p(out)), "phx_check_crmath")
     return out
+
+
+class Tuning(C.Structure):
+    """PhxTuning of include/phygas_hip.h."""
+    _fields_ = [("scan_blocks", C.c_int), ("items_blocks", C.c_int), ("reduce_blocks", C.c_int),
+                ("finish_blocks", C.c_int), ("stars_per_task", C.c_int), ("coarse_tiles", C.c_int),
+                ("pass_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class WuOptions(C.Structure):
@@ -293,6 +307,23 @@ class Engine:
         return {f"{k}_blocks": int(g) for k, g in zip(("scan", "items", "reduce", "finish"), grids)} | \
             {f"{k}_ms": float(t) for k, t in zip(("scan", "items", "reduce", "finish"), ms)}
 
+    def autotune_ex(self, form: int = FAST, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> dict:
+        """Every tunable setting of the fast pass (the scan's first-level time tile, stars per task, the four
+        launch grids) chosen by timing whole passes on the loaded batch; the choices stay set."""
+        t = Tuning()
+        _check(self._L.phx_autotune_ex(self.h, form, chi2_max, n_samp, p_min, C.byref(t)), "phx_autotune_ex")
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(self._L.phx_get_tuning(self.h, C.byref(t)), "phx_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, scan: int = 0, items: int = 0, reduce: int = 0, finish: int = 0, stars_per_task: int = 16,
+                   coarse_tiles: int = 1):
+        t = Tuning(scan, items, reduce, finish, stars_per_task, coarse_tiles, 0.0)
+        _check(self._L.phx_set_tuning(self.h, C.byref(t)), "phx_set_tuning")
+
     def set_grids(self, scan: int = 0, items: int = 0, reduce: int = 0, finish: int = 0):
         """Launch grids of the fast pass's scan, items, reduce and finish (0: the library's default)."""
         g = np.array([scan, items, reduce, finish], np.int32)
@@ -395,6 +426,18 @@ class Engine:
         _check(self._L.phx_pair_loaded(self.h, form, len(star), _p(star), _p(cluster), n_clones, dt_myr, _p(res)),
                "phx_pair_loaded")
         return res
+
+    def set_pair_bucket(self, chunks: int):
+        _check(self._L.phx_set_pair_bucket(self.h, chunks), "phx_set_pair_bucket")
+
+    def autotune_pairs(self, star, cluster, n_clones: int, dt_myr: float = 0.05, form: int = FAST) -> dict:
+        """Bucket width of the clone kernel chosen by timing this batch (phx_autotune_pairs)."""
+        star = np.ascontiguousarray(star, dtype=np.int32)
+        cluster = np.ascontiguousarray(cluster, dtype=np.int32)
+        b, ms = _I(0), _D(0)
+        _check(self._L.phx_autotune_pairs(self.h, form, len(star), _p(star), _p(cluster), n_clones, dt_myr,
+                                          C.byref(b), C.byref(ms)), "phx_autotune_pairs")
+        return {"bucket": b.value, "ms": ms.value}
 
     def pair_graph_stats(self) -> tuple[int, int]:
         c, r = _L(0), _L(0)

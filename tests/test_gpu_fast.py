@@ -221,27 +221,47 @@ def test_full_work_unit_fast_and_fast64_against_plain(workload):
 
 
 def test_autotune_keeps_rows_and_stats_consistent(unit):
-    """Launch grids chosen by the autotuner change no row; the pass's work counts are consistent."""
+    """Settings chosen by the autotuner (the scan's first-level time tile, stars per task, launch grids) change no
+    row; every combination of the scan settings gives the same rows, and the pass's work counts follow the
+    levels it ran."""
     cl, st, ctl = unit
+    cn = np.ceil(cl["tmax_myr"] / 0.5).astype(np.int64) + 1
+    fine_tiles = int(np.sum((cn + 7) // 8))
+    coarse_tiles = int(np.sum(((cn + 7) // 8 + 1) // 2))
     with hip.Engine(0) as e:
         e.load_clusters(cl)
         e.load_stars(st, 0)
         e.load_stars(ctl, 1)
         before = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
-        stats = e.pass_stats()
-        tune = e.autotune(hip.FAST, 25.0, 64, 0.0)
+        plain = e.screen(hip.PLAIN, chi2_max=25.0, p_min=0.0)
+        tune = e.autotune_ex(hip.FAST, 25.0, 64, 0.0)
         after = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
-        stats2 = e.pass_stats()
-    assert before.tobytes() == after.tobytes()
-    assert stats == stats2
-    cn = np.ceil(cl["tmax_myr"] / 0.5).astype(np.int64) + 1
-    # level 1 runs on coarse tiles (two tiles each); level 2 on those and on the tiles of the coarse ones left
-    assert stats["l1_tests"] == (len(st) + len(ctl)) * int(np.sum(((cn + 7) // 8 + 1) // 2))
-    assert stats["candidates"] <= stats["items"] <= stats["l2_tests"] <= 3 * stats["l1_tests"]
-    assert stats["rows"] == len(before) and stats["kept"] >= stats["rows"]
-    for k in ("scan", "items", "reduce", "finish"):
-        assert tune[f"{k}_blocks"] > 0 and tune[f"{k}_ms"] > 0
-        _report(f"autotune_{k}_blocks", tune[f"{k}_blocks"])
+        assert before.tobytes() == after.tobytes()
+        for k in ("scan", "items", "reduce", "finish"):
+            assert tune[f"{k}_blocks"] > 0
+            _report(f"autotune_{k}_blocks", tune[f"{k}_blocks"])
+        _report("autotune_stars_per_task", tune["stars_per_task"])
+        _report("autotune_coarse_tiles", tune["coarse_tiles"])
+        assert tune["stars_per_task"] in (8, 16) and tune["coarse_tiles"] in (0, 1) and tune["pass_ms"] > 0
+        seen = {}
+        for coarse in (1, 0):
+            for spw in (16, 8):
+                e.set_tuning(stars_per_task=spw, coarse_tiles=coarse)
+                rows = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+                stats = e.pass_stats()
+                assert rows.tobytes() == before.tobytes(), (coarse, spw)
+                assert e.screen(hip.FAST64, chi2_max=25.0, p_min=0.0).tobytes() == plain.tobytes(), (coarse, spw)
+                n = len(st) + len(ctl)
+                # level 1 on coarse tiles (two tiles each) or on every tile; level 2 on its survivors (and, with the
+                # coarse level, on the tiles of the coarse ones left)
+                assert stats["l1_tests"] == n * (coarse_tiles if coarse else fine_tiles)
+                assert stats["candidates"] <= stats["items"] <= stats["l2_tests"] <= 3 * stats["l1_tests"]
+                assert stats["rows"] == len(before) and stats["kept"] >= stats["rows"]
+                if coarse in seen:   # stars per task change no test
+                    assert stats == seen[coarse]
+                seen[coarse] = stats
+                _report(f"scan_l1_tests_coarse{coarse}", stats["l1_tests"])
+                _report(f"scan_items_coarse{coarse}", stats["items"])
 
 
 @pytest.mark.parametrize("ns", [0, 1, 17, 4100])

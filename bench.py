@@ -199,15 +199,26 @@ def monte_carlo(e: hip.Engine, clusters, pulsar, reps: int = 5) -> dict:
     for name, form, flop in (("plain", hip.PLAIN, PLAIN_CLONE_STEP_FLOP), ("fast", hip.FAST, FAST_CLONE_STEP_FLOP)):
         res[name] = e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True)
         ms = _median_ms(lambda: e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True), reps)
+        lanes = e.pair_lanes(len(ci) * n_clones, form)
         e.set_timing(True)
-        e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True)
-        st = e.stage_times()
+        by_lanes = {}
+        for setting in (1, 2, 0):   # the clone kernel alone at each lane count, then as the library chooses
+            e.set_pair_lanes(setting)
+            kt = []
+            for _ in range(3):
+                assert e.pair(pulsar, clusters, ti, ci, n_clones, dt, form=form, device_draws=True).tobytes() == \
+                    res[name].tobytes()
+                kt.append(e.stage_times())
+            if setting:
+                by_lanes[str(setting)] = float(np.median([t.get("clones", float("nan")) for t in kt]))
+        st = kt[len(kt) // 2]
         e.set_timing(False)
-        kms = st.get("clones", float("nan"))
+        kms = float(np.median([t.get("clones", float("nan")) for t in kt]))
         achieved = steps * flop / (kms / 1e3) / 1e12
         out[name] = {"ms_per_job": ms, "clone_steps_per_s": steps / (ms / 1e3), "stages_ms": st,
-                     "clone_kernel_clone_steps_per_s": steps / (kms / 1e3),
-                     "roofline": {"bound": "valu", "kernel": "k_clones<%d>" % int(form == hip.FAST),
+                     "clone_kernel_clone_steps_per_s": steps / (kms / 1e3), "lanes_per_clone": lanes,
+                     "clone_kernel_ms_by_lanes": by_lanes,
+                     "roofline": {"bound": "valu", "kernel": "k_clones<%d, %d>" % (int(form == hip.FAST), lanes),
                                   "achieved": achieved, "peak": PEAK_VALU, "unit": "TFLOP/s",
                                   "frac": achieved / PEAK_VALU, "work_per_launch": steps * flop,
                                   "flop_per_clone_step": flop, "avg_launch_ms": kms}}
@@ -251,6 +262,8 @@ def two_stage(clusters, n_stars: int = 100000, n_clones: int = 1000, p_stage2: f
             out[f"stage2_{name}_ms"] = ms
             out[f"stage2_{name}_clone_steps_per_s"] = steps / (ms / 1e3)
         cap, rep = e.pair_graph_stats()
+        out["stage2_lanes_per_clone"] = {"fast": e.pair_lanes(len(si) * n_clones, hip.FAST),
+                                         "plain": e.pair_lanes(len(si) * n_clones, hip.PLAIN)}
         # stage 2 at the config's expected scale (~60,000 pairs): the real rows with the highest p_r50
         big = rows[rows["flags"] == 0]
         big = big[np.argsort(-big["p_r50"], kind="stable")][:60000]

@@ -177,12 +177,20 @@ int phx_pair_loaded(PhxCtx *ctx, int form, int npairs, const int *star, const in
                     double dt_myr, PhResult *res);
 /* Captures and replays of the device-draw Monte Carlo graphs. */
 int phx_pair_graph_stats(PhxCtx *ctx, long *captures, long *replays);
-/* Bucket width of the persistent clone kernel: 64-clone chunks per queue grab (1..64; default 1).  Wider buckets
- * take fewer queue atomics, narrower ones balance the tail better; results do not depend on it.  No
- * counterpart in the reference. */
+/* Bucket width of the persistent clone kernel: chunks (one wavefront's clones) per queue grab (1..64; default 1).
+ * Wider buckets take fewer queue atomics, narrower ones balance the tail better; results do not depend on it.
+ * No counterpart in the reference. */
 int phx_set_pair_bucket(PhxCtx *ctx, int chunks);
-/* The bucket width chosen by timing phx_pair_loaded on the caller's batch over 1, 2, 4 and 8 chunks; it stays set
- * on the context.  *bucket, *ms: the choice and its time.  No counterpart in the reference. */
+/* Lanes per clone of the clone kernel: 1 (target and cluster in one lane), 2 (one lane each), 0 (default: 1 for
+ * the FP32 form; for FP64, 2 below 2 x 64 x SIMDs clones, else 1).  Results do not depend on it.  No
+ * counterpart in the reference. */
+int phx_set_pair_lanes(PhxCtx *ctx, int lanes);
+/* The lanes per clone (1 or 2) a batch of n_clones clone slots of the given form runs with under the context's
+ * setting; -1 for an unknown form. */
+int phx_pair_lanes_for(PhxCtx *ctx, int form, long n_clones);
+/* Lanes per clone and bucket width chosen by timing phx_pair_loaded on the caller's batch over 1 and 2 lanes x 1,
+ * 2, 4 and 8 chunks; both stay set on the context.  *bucket, *ms: the bucket chosen and the time.  No counterpart
+ * in the reference. */
 int phx_autotune_pairs(PhxCtx *ctx, int form, int npairs, const int *star, const int *cluster, int n_clones,
                        double dt_myr, int *bucket, double *ms);
 

@@ -92,7 +92,8 @@ struct PhxCtx {
     hipGraphExec_t pgexec = nullptr;
     uint64_t pgkey = 0;
     long pcaptures = 0, preplays = 0;
-    int pair_bucket = 1;   // 64-clone chunks per queue grab of the clone kernel (phx_set_pair_bucket)
+    int pair_bucket = 1;   // chunks per queue grab of the clone kernel (phx_set_pair_bucket)
+    int pair_lanes = 0;    // lanes per clone, 0 = chosen from the batch (phx_set_pair_lanes)
 };
 
 static FastWork &part_fw(PhxCtx *c, int i) { return i == 0 ? c->fw : c->pfw[i]; }
@@ -780,7 +781,8 @@ extern "C" int phx_pair_clones(PhxCtx *c, int form, const double *ps, const int 
     PHX_CK(hipMemcpyAsync(L.steps, steps, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(L.dts, dts, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
     stage_begin(c);
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, n, c->pair_bucket, L.next, c->st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, n, c->pair_bucket, c->pair_lanes, L.next, c->st))
+        return 1;
     stage_mark(c, "clones");
     std::vector<double> r(3 * (size_t)n);
     PHX_CK(hipMemcpyAsync(r.data(), L.res, 24 * (size_t)n, hipMemcpyDeviceToHost, c->st));
@@ -805,7 +807,7 @@ static int pair_run_launches(PhxCtx *c, const PairLayout &L, const PairPlan &pla
     uint64_t key = FNV_BASIS;
     auto mix = [&](uint64_t v) { key = (key ^ v) * FNV_PRIME; };
     mix((uint64_t)plan.npairs); mix((uint64_t)plan.n_clones); mix((uint64_t)plan.slots); mix((uint64_t)form);
-    mix((uint64_t)plan.bucket);
+    mix((uint64_t)plan.bucket); mix((uint64_t)plan.lanes);
     mix((uint64_t)(uintptr_t)L.ps); mix((uint64_t)(uintptr_t)d_t); mix((uint64_t)(uintptr_t)d_c);
     if (!c->pgexec || c->pgkey != key) {
         if (c->pgexec) (void)hipGraphExecDestroy(c->pgexec);
@@ -840,6 +842,7 @@ static int pair_device_draws(PhxCtx *c, int form, const PhTarget *t, const PhClu
     PairPlan plan;
     pair_plan(t, cl, npairs, ti, ci, n_clones, dt_myr, plan);
     plan.bucket = c->pair_bucket;
+    plan.lanes = c->pair_lanes;
     PairLayout L;
     if (pair_buffers(c, plan.slots, npairs, nt, ncl, L)) return 1;
     PHX_CK(hipMemcpyAsync(L.tg, t, sizeof(PhTarget) * nt, hipMemcpyHostToDevice, c->st));
@@ -873,6 +876,7 @@ extern "C" int phx_pair_loaded(PhxCtx *c, int form, int npairs, const int *star,
     PairPlan plan;
     pair_plan(c->st_h.data(), c->cl_h.data(), npairs, star, cluster, n_clones, dt_myr, plan);
     plan.bucket = c->pair_bucket;
+    plan.lanes = c->pair_lanes;
     PairLayout L;
     if (pair_buffers(c, plan.slots, npairs, 0, 0, L)) return 1;
     if (pair_upload_plan(L, plan, star, cluster, c->st)) return 1;
@@ -883,35 +887,51 @@ extern "C" int phx_pair_loaded(PhxCtx *c, int form, int npairs, const int *star,
 }
 
 extern "C" int phx_set_pair_bucket(PhxCtx *c, int chunks) {
-    PHX_REQUIRE(chunks >= 1 && chunks <= 64, "bucket width must be 1..64 chunks of 64 clones");
+    PHX_REQUIRE(chunks >= 1 && chunks <= 64, "bucket width must be 1..64 chunks");
     c->pair_bucket = chunks;
     return 0;
 }
 
-// The clone kernel's bucket width chosen by timing phx_pair_loaded on the caller's batch (results do not depend
-// on it); the choice stays set.  No counterpart in the reference.
+extern "C" int phx_set_pair_lanes(PhxCtx *c, int lanes) {
+    PHX_REQUIRE(lanes >= 0 && lanes <= 2, "lanes per clone must be 0 (chosen from the batch), 1 or 2");
+    c->pair_lanes = lanes;
+    return 0;
+}
+
+// The clone kernel's lanes per clone and bucket width chosen by timing phx_pair_loaded on the caller's batch
+// (results do not depend on them); the choice stays set.  No counterpart in the reference.
 extern "C" int phx_autotune_pairs(PhxCtx *c, int form, int npairs, const int *star, const int *cluster, int n_clones,
                                   double dt_myr, int *bucket, double *ms) {
     PHX_REQUIRE(npairs > 0, "no pairs to tune on");
     std::vector<PhResult> res(npairs);
     const int opts[4] = {1, 2, 4, 8};
-    int best = c->pair_bucket;
+    int best = c->pair_bucket, best_lanes = c->pair_lanes;
     double best_ms = 1e300;
-    for (int o : opts) {
-        c->pair_bucket = o;
-        double t = 1e300;
-        for (int rep = 0; rep < 3; rep++) {   // the first call with a new width is not counted
-            const auto t0 = std::chrono::steady_clock::now();
-            if (int rc = phx_pair_loaded(c, form, npairs, star, cluster, n_clones, dt_myr, res.data())) return rc;
-            const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (rep > 0) t = std::min(t, dt);
+    for (int lanes = 1; lanes <= 2; lanes++)
+        for (int o : opts) {
+            c->pair_bucket = o;
+            c->pair_lanes = lanes;
+            double t = 1e300;
+            for (int rep = 0; rep < 3; rep++) {   // the first call with a new setting is not counted
+                const auto t0 = std::chrono::steady_clock::now();
+                if (int rc = phx_pair_loaded(c, form, npairs, star, cluster, n_clones, dt_myr, res.data())) return rc;
+                const double dt =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (rep > 0) t = std::min(t, dt);
+            }
+            if (t < best_ms) { best_ms = t; best = o; best_lanes = lanes; }
         }
-        if (t < best_ms) { best_ms = t; best = o; }
-    }
     c->pair_bucket = best;
+    c->pair_lanes = best_lanes;
     if (bucket) *bucket = best;
     if (ms) *ms = best_ms;
     return 0;
+}
+
+extern "C" int phx_pair_lanes_for(PhxCtx *c, int form, long n_clones) {
+    if (check_pair_args(form, 0, 0)) return -1;
+    PHX_CK(hipSetDevice(c->device));
+    return clone_lanes(form, n_clones, c->pair_lanes);
 }
 
 extern "C" int phx_pair_graph_stats(PhxCtx *c, long *captures, long *replays) {
@@ -950,7 +970,9 @@ extern "C" int phx_pair(PhxCtx *c, int form, const PhTarget *t, const PhCluster 
     PHX_CK(hipMemcpyAsync(L.dts, dts.data(), 8 * n, hipMemcpyHostToDevice, c->st));
     PHX_CK(hipMemcpyAsync(L.m, m.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice, c->st));
     stage_begin(c);
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, plan.slots, c->pair_bucket, L.next, c->st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, c->P, plan.slots, c->pair_bucket, c->pair_lanes, L.next,
+                      c->st))
+        return 1;
     stage_mark(c, "clones");
     if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, npairs, n_clones, L.out, c->st)) return 1;
     stage_mark(c, "reduce");

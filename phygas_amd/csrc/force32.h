@@ -10,7 +10,8 @@
 //                   reference (reference: src/phygas.c:286-323: six orbits 1e-5 apart, differenced) to FP32
 //                   relative precision of the offsets, not the tangent-linear approximation (which differs
 //                   from the finite difference by ~1e-4 relative after 100 Myr).
-// * float-float:    TwoSum / Fast2Sum updates for the Kahan-compensated position and velocity of the nominal.
+// * ff_add():       float-float (Kahan-compensated) accumulation, for host measurements of an FP32 nominal orbit
+//                   with compensated position and velocity (the kernels keep the nominal in FP64).
 //
 // Potential: reference src/potential.c:54-101.
 #pragma once

@@ -42,7 +42,8 @@ PairLayout pair_layout(char *base, long slots, int npairs, int ntg, int ncl);
 // radii, and slot offsets with the pairs ranked longest trace-back first (n_clones slots each).
 struct PairPlan {
     int npairs = 0, n_clones = 0, max_steps = 0;
-    int bucket = 1;   // 64-clone chunks per queue grab of the clone kernel
+    int bucket = 1;   // chunks (one wavefront's clones) per queue grab of the clone kernel
+    int lanes = 0;    // lanes per clone: 1, 2, or 0 (chosen from the form and the batch size)
     long slots = 0;
     std::vector<uint64_t> seed;
     std::vector<int> psteps;
@@ -61,9 +62,13 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
 int pair_launch_draws(const PairLayout &L, const PairPlan &plan, const PhTarget *d_t, const PhCluster *d_c,
                       const Pot &P, hipStream_t st);
 // Traces of n clones (persistent, queue counter d_next zeroed by a kernel first; each queue grab takes cw
-// 64-clone chunks).  form: PHX_FORM_FAST = FP32 force, otherwise FP64.
+// chunks of one wavefront's clones).  form: PHX_FORM_FAST = FP32 force, otherwise FP64.  lanes per clone: 1,
+// 2 or 0 (chosen from n); the results do not depend on it.
 int launch_clones(int form, const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P,
-                  long n, int cw, unsigned *d_next, hipStream_t st);
+                  long n, int cw, int lanes, unsigned *d_next, hipStream_t st);
+// The lanes per clone a launch of n clones of the given form takes for the setting lanes (0: one for the FP32
+// form; for FP64 two below 2 x 64 x SIMDs clones, else one).
+int clone_lanes(int form, long n, int lanes);
 // Quantiles and fractions of each pair's clones; max_m bounds every m[p] (chooses the LDS sort or selection).
 int launch_pair_reduce(const double *d_res, const long *d_off, const int *d_m, const double *d_r50_pc,
                        const double *d_rtot_pc, int npairs, int max_m, PhResult *d_out, hipStream_t st);

@@ -5,10 +5,11 @@
 //   1. k_pair_draw    clone start states drawn on the device from each pair's splitmix64 stream (or uploaded
 //                     host draws), pair p owning the clone slots [off[p], off[p] + m[p])
 //   2. k_pair_steps   every slot's step count and step (0 steps for unused slots)
-//   3. k_clones       the traces: one lane per clone, a persistent grid whose wavefronts take 64-clone chunks
-//                     from one queue in slot order; pairs are laid out longest trace-back first, so the longest
-//                     chunks start first and the short ones fill the tail, and a chunk's lanes (neighbouring
-//                     clones of one pair or of two pairs of similar length) run nearly the same number of steps
+//   3. k_clones       the traces: one lane per clone (two, target and cluster, for small batches), a persistent
+//                     grid whose wavefronts take chunks of 64 (32) clones from one queue in slot order; pairs are
+//                     laid out longest trace-back first, so the longest chunks start first and the short ones
+//                     fill the tail, and a chunk's lanes (neighbouring clones of one pair or of two pairs of
+//                     similar length) run nearly the same number of steps
 //   4. reduce         per pair: counts within r50 / rtot and the 5/50/95 % quantiles of distance, time and
 //                     speed, exactly as the reference's sort-and-interpolate (LDS bitonic sort up to 4,096
 //                     clones, an 8-pass radix selection of the six order statistics above that)
@@ -184,23 +185,85 @@ __device__ __forceinline__ void trace_clone(const double *__restrict__ p, int ns
     R[2] = vbest;
 }
 
-// Persistent grid: each wavefront takes the next bucket of cw 64-clone chunks from *next (zeroed before the
-// launch) and traces them one after another, until the buckets run out; every wavefront reaches that exit.
-// Wider buckets take fewer queue atomics, narrower ones balance the tail better (the autotuner picks cw).
+// Two lanes per clone: the even lane steps the target, the odd lane the cluster, with the operations of
+// trace_clone for that orbit.  After each step the lanes swap positions (and, at a new minimum, velocities) and
+// both evaluate the closest approach from the same values, so both take every branch together and the even
+// lane holds trace_clone's bits.  Half the instructions per lane and twice the wavefronts: for FP64 batches too
+// small to give every SIMD several wavefronts at one lane per clone.
+// The value of the neighbouring lane (lane ^ 1): a DPP quad permutation [1, 0, 3, 2] of each 32-bit half, a VALU
+// move with no LDS round trip.  Both lanes of a clone are active wherever it is used.
+__device__ __forceinline__ double swap_lane_pair(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0xB1, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0xB1, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, (uint64_t)lo | (uint64_t)hi << 32);
+}
+
 template <int Fast>
+__device__ __forceinline__ void trace_clone2(const double *__restrict__ p, int ns, double dt, const Pot &P,
+                                             const PotF &Q, int side, double *__restrict__ R) {
+    auto other = [](V3 a) { return v3(swap_lane_pair(a.x), swap_lane_pair(a.y), swap_lane_pair(a.z)); };
+    auto tmc = [side](V3 own, V3 oth) { return side == 0 ? own - oth : oth - own; };   // target minus cluster
+    const double *q = p + 6 * side;
+    V3 x = v3(q[0], q[1], q[2]), v = v3(q[3], q[4], q[5]);
+    Body b;
+    F3 f = {0.0f, 0.0f, 0.0f};
+    if (Fast) f = force32(Q, (float)x.x, (float)x.y, (float)x.z);
+    else b = body_start(P, x, v);
+    V3 r0 = tmc(x, other(x));
+    double best = norm(r0), tbest = 0.0, vbest = norm(tmc(v, other(v)));
+    const double h = 0.5 * dt;
+    for (int i = 1; i <= ns; i++) {
+        if (Fast) {
+            v = v + v3(f.x, f.y, f.z) * h;
+            x = x + v * dt;
+            f = force32(Q, (float)x.x, (float)x.y, (float)x.z);
+            v = v + v3(f.x, f.y, f.z) * h;
+        } else {
+            kdk(P, b, dt);
+            x = b.pos;
+            v = b.vel;
+        }
+        const V3 r1 = tmc(x, other(x));
+        const double s = seg_param(r0, r1);
+        const double dm = norm(r0 + (r1 - r0) * s);
+        if (dm < best) {   // the same decision in both lanes of the clone
+            best = dm;
+            tbest = (i - 1 + s) * -dt;
+            vbest = norm(tmc(v, other(v)));
+        }
+        r0 = r1;
+    }
+    if (side == 0) {
+        R[0] = best;
+        R[1] = tbest;
+        R[2] = vbest;
+    }
+}
+
+// Persistent grid: each wavefront takes the next bucket of cw chunks (64 / LPC clones each, LPC lanes per
+// clone) from *next (zeroed before the launch) and traces them one after another, until the buckets run out;
+// every wavefront reaches that exit.  Wider buckets take fewer queue atomics, narrower ones balance the tail
+// better (the autotuner picks cw and LPC).
+template <int Fast, int LPC>
 __global__ __launch_bounds__(256) void k_clones(const double *__restrict__ ps, const int *__restrict__ steps,
                                                 const double *__restrict__ dts, double *__restrict__ res, Pot P,
                                                 PotF Q, long n, int cw, unsigned *__restrict__ next) {
+    static_assert(LPC == 1 || LPC == 2, "one or two lanes per clone");
+    constexpr int CPC = 64 / LPC;   // clones per chunk
     const int lane = threadIdx.x & 63;
-    const long nchunks = (n + 63) / 64, nbuckets = (nchunks + cw - 1) / cw;
+    const long nchunks = (n + CPC - 1) / CPC, nbuckets = (nchunks + cw - 1) / cw;
     for (;;) {
         unsigned b = 0;
         if (lane == 0) b = atomicAdd(next, 1u);
         b = __shfl(b, 0, 64);
         if ((long)b >= nbuckets) break;
         for (int u = 0; u < cw; u++) {
-            const long k = ((long)b * cw + u) * 64 + lane;
-            if (k < n) trace_clone<Fast>(ps + (size_t)k * 12, steps[k], dts[k], P, Q, res + (size_t)k * 3);
+            const long k = ((long)b * cw + u) * CPC + lane / LPC;   // the same k in both lanes of a clone
+            if (k < n) {
+                if (LPC == 1) trace_clone<Fast>(ps + (size_t)k * 12, steps[k], dts[k], P, Q, res + (size_t)k * 3);
+                else trace_clone2<Fast>(ps + (size_t)k * 12, steps[k], dts[k], P, Q, lane & 1, res + (size_t)k * 3);
+            }
         }
     }
 }
@@ -371,39 +434,63 @@ __global__ __launch_bounds__(256) void k_pair_reduce_select(const double *__rest
 }
 
 // ---------------------------------------------------------------- launches
-static int clone_grid(int fast, long n, int cw) {
-    static int per_cu[2] = {0, 0}, cus = 0;
+// Lanes per clone when the caller leaves it open (lanes = 0).  A small batch's kernel time is the longest
+// clone's chain of steps.  The FP32 form's step is latency-bound: one lane already overlaps the two orbits'
+// forces, and two lanes only add the lane exchange to the chain, so it keeps one.  The FP64 step is long
+// enough to be issue-bound for a wavefront alone on its SIMD: two lanes halve it, while one lane per clone
+// would give the device fewer than two wavefronts per SIMD.
+int clone_lanes(int form, long n, int lanes) {
+    if (lanes == 1 || lanes == 2) return lanes;
+    if (form == PHX_FORM_FAST) return 1;
+    static int simds = 0;
+    if (!simds) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        simds = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess
+                    ? 4 * prop.multiProcessorCount
+                    : 1024;
+    }
+    return n < 2L * 64 * simds ? 2 : 1;
+}
+
+template <int Fast, int LPC>
+static int clone_blocks_per_cu() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_clones<Fast, LPC>, 256, 0) != hipSuccess) return 0;
+    return std::max(b, 1);
+}
+
+static int clone_grid(int fast, int lpc, long n, int cw) {
+    static int per_cu[2][2] = {{0, 0}, {0, 0}}, cus = 0;
     if (!cus) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 0;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+        per_cu[0][0] = clone_blocks_per_cu<0, 1>();
+        per_cu[0][1] = clone_blocks_per_cu<0, 2>();
+        per_cu[1][0] = clone_blocks_per_cu<1, 1>();
+        per_cu[1][1] = clone_blocks_per_cu<1, 2>();
+        if (!per_cu[0][0] || !per_cu[0][1] || !per_cu[1][0] || !per_cu[1][1]) return 0;
         cus = prop.multiProcessorCount;
-        int b0 = 0, b1 = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, k_clones<0>, 256, 0) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_clones<1>, 256, 0) != hipSuccess)
-            return 0;
-        per_cu[0] = std::max(b0, 1);
-        per_cu[1] = std::max(b1, 1);
     }
-    const long buckets = ((n + 63) / 64 + cw - 1) / cw, blocks_needed = (buckets + 3) / 4;
-    return (int)std::min<long>((long)per_cu[fast] * cus, std::max(blocks_needed, 1L));
+    const int cpc = 64 / lpc;
+    const long buckets = ((n + cpc - 1) / cpc + cw - 1) / cw, blocks_needed = (buckets + 3) / 4;
+    return (int)std::min<long>((long)per_cu[fast][lpc - 1] * cus, std::max(blocks_needed, 1L));
 }
 
 int launch_clones(int form, const double *d_ps, const int *d_steps, const double *d_dts, double *d_res, const Pot &P,
-                  long n, int cw, unsigned *d_next, hipStream_t st) {
+                  long n, int cw, int lanes, unsigned *d_next, hipStream_t st) {
     if (n <= 0) return 0;
     PHX_REQUIRE(cw >= 1 && cw <= 64, "clone bucket width must be 1..64 chunks");
-    const int fast = form == PHX_FORM_FAST ? 1 : 0;
-    const int grid = clone_grid(fast, n, cw);
+    PHX_REQUIRE(lanes >= 0 && lanes <= 2, "lanes per clone must be 0 (chosen), 1 or 2");
+    const int fast = form == PHX_FORM_FAST ? 1 : 0, lpc = clone_lanes(form, n, lanes);
+    const int grid = clone_grid(fast, lpc, n, cw);
     PHX_REQUIRE(grid > 0, "cannot size the clone grid");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, d_next);
-    if (fast)
-        hipLaunchKernelGGL(k_clones<1>, dim3(grid), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res, P, make_potf(P), n,
-                           cw, d_next);
-    else
-        hipLaunchKernelGGL(k_clones<0>, dim3(grid), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res, P, make_potf(P), n,
-                           cw, d_next);
+    const PotF Q = make_potf(P);
+    auto kern = fast ? (lpc == 2 ? k_clones<1, 2> : k_clones<1, 1>) : (lpc == 2 ? k_clones<0, 2> : k_clones<0, 1>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, d_ps, d_steps, d_dts, d_res, P, Q, n, cw, d_next);
     PHX_CK(hipGetLastError());
     return 0;
 }
@@ -480,7 +567,8 @@ int pair_launch_device(const PairLayout &L, const PairPlan &plan, int form, cons
     if (np <= 0) return 0;
     if (pair_launch_draws(L, plan, d_t, d_c, P, st)) return 1;
     if (mark) mark(mark_ctx, "draws");
-    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, P, plan.slots, plan.bucket, L.next, st)) return 1;
+    if (launch_clones(form, L.ps, L.steps, L.dts, L.res, P, plan.slots, plan.bucket, plan.lanes, L.next, st))
+        return 1;
     if (mark) mark(mark_ctx, "clones");
     if (launch_pair_reduce(L.res, L.off, L.m, L.r50, L.rt, np, plan.n_clones, L.out, st)) return 1;
     if (mark) mark(mark_ctx, "reduce");

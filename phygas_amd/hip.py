@@ -81,6 +81,8 @@ def lib() -> C.CDLL:
         "phx_pair_noalloc_limit": (C.c_size_t, []),
         "phx_pair_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_pair_bucket": (_I, [_P, _I]),
+        "phx_set_pair_lanes": (_I, [_P, _I]),
+        "phx_pair_lanes_for": (_I, [_P, _I, _L]),
         "phx_autotune_pairs": (_I, [_P, _I, _I, _P, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "phx_set_split": (_I, [_P, _I]),
@@ -430,14 +432,22 @@ class Engine:
     def set_pair_bucket(self, chunks: int):
         _check(self._L.phx_set_pair_bucket(self.h, chunks), "phx_set_pair_bucket")
 
+    def set_pair_lanes(self, lanes: int):
+        """Lanes per clone of the clone kernel: 1, 2, or 0 (chosen from the batch)."""
+        _check(self._L.phx_set_pair_lanes(self.h, lanes), "phx_set_pair_lanes")
+
+    def pair_lanes(self, n_clones: int, form: int = PLAIN) -> int:
+        """Lanes per clone a batch of n_clones clone slots of this form runs with under the current setting."""
+        return int(self._L.phx_pair_lanes_for(self.h, form, n_clones))
+
     def autotune_pairs(self, star, cluster, n_clones: int, dt_myr: float = 0.05, form: int = FAST) -> dict:
-        """Bucket width of the clone kernel chosen by timing this batch (phx_autotune_pairs)."""
+        """Lanes per clone and bucket width of the clone kernel chosen by timing this batch (phx_autotune_pairs)."""
         star = np.ascontiguousarray(star, dtype=np.int32)
         cluster = np.ascontiguousarray(cluster, dtype=np.int32)
         b, ms = _I(0), _D(0)
         _check(self._L.phx_autotune_pairs(self.h, form, len(star), _p(star), _p(cluster), n_clones, dt_myr,
                                           C.byref(b), C.byref(ms)), "phx_autotune_pairs")
-        return {"bucket": b.value, "ms": ms.value}
+        return {"bucket": b.value, "lanes": self.pair_lanes(len(star) * n_clones, form), "ms": ms.value}
 
     def pair_graph_stats(self) -> tuple[int, int]:
         c, r = _L(0), _L(0)

@@ -5,6 +5,7 @@
 * Pairs of the loaded stars and resident clusters (phx_pair_loaded, device draws from the resident records)
   give the same results as the same pairs with host-given records (phx_pair_ex, device draws).
 * Graph replays of a batch equal its eager run.
+* One and two lanes per clone, any bucket width and the autotuned choice give the same bits.
 * The two-stage shard on a reduced sample (600 stars of seed 4): stage 1 screens, stage 2 runs the exact Monte
   Carlo with 1,000 clones per pair for every row with p_r50 >= 0.01; against the oracle's ph_pair on the same
   pairs.  Bar: quantiles within 1e-3 relative (the Monte Carlo bar; the plain form's measured maxima go to
@@ -116,3 +117,55 @@ def test_two_stage_shard_against_the_oracle(workload, oracle):
     df = np.maximum(np.abs(fast["f_r50"] - want["f_r50"]), np.abs(fast["f_rtot"] - want["f_rtot"]))
     _report("two_stage_fast_f_max_diff", df.max())
     assert df.max() <= 2.0 / 1000      # at most a clone or two on a radius
+
+
+def test_lanes_per_clone_and_buckets_give_the_same_bits(oracle):
+    """One lane per clone and two (target and cluster orbits in neighbouring lanes), with several bucket widths:
+    every clone's closest approach, its time and speed bit for bit the same, in both forms; the plain form's
+    against the oracle's clones within 1e-12.  Clones of three clusters of different trace-back lengths, so
+    that the lanes of a wavefront run different step counts."""
+    cl = pio.read_clusters(os.path.join(INP, "clusters400.txt"))
+    pul = pio.read_targets(os.path.join(INP, "pulsar1.txt"))
+    order = np.argsort(cl["tmax_myr"])
+    picks = [int(order[len(order) // 10]), int(order[len(order) // 2]), int(order[-1])]
+    ps, steps, dts = [], [], []
+    for k, i in enumerate(picks):
+        p, n, dt = oracle.pair_draw(pul[0], cl[i], 150 + 40 * k, 0.05, oracle.seed("lanes", str(i)))
+        ps.append(p)
+        steps.append(np.full(len(p), n, np.int32))
+        dts.append(np.full(len(p), dt))
+    ps, steps, dts = np.concatenate(ps), np.concatenate(steps), np.concatenate(dts)
+    assert len(set(steps.tolist())) == 3
+    want = oracle.pair_clones(ps, steps, dts)
+    with hip.Engine(0) as e:
+        for form in (hip.PLAIN, hip.FAST):
+            e.set_pair_lanes(1)
+            e.set_pair_bucket(1)
+            base = e.pair_clones(ps, steps, dts, form=form)
+            if form == hip.PLAIN:
+                for a, b in zip(base, want):
+                    assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)) <= 1e-12
+            for lanes, bucket in ((2, 1), (2, 3), (1, 3), (0, 2)):
+                e.set_pair_lanes(lanes)
+                e.set_pair_bucket(bucket)
+                got = e.pair_clones(ps, steps, dts, form=form)
+                for a, b in zip(got, base):
+                    assert a.tobytes() == b.tobytes(), (form, lanes, bucket)
+        e.set_pair_lanes(0)
+        e.set_pair_bucket(1)
+
+
+def test_autotuned_pairs_keep_results(workload):
+    cl = workload.make_clusters(600, 0)
+    st = workload.make_stars(20, 4)
+    rng = np.random.default_rng(4)
+    si = rng.integers(0, len(st), 40).astype(np.int32)
+    ci = rng.integers(0, len(cl), 40).astype(np.int32)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st)
+        before = e.pair_loaded(si, ci, n_clones=250, form=hip.FAST)
+        t = e.autotune_pairs(si, ci, n_clones=250, form=hip.FAST)
+        assert t["lanes"] in (1, 2) and t["bucket"] in (1, 2, 4, 8) and t["ms"] > 0
+        after = e.pair_loaded(si, ci, n_clones=250, form=hip.FAST)
+    assert after.tobytes() == before.tobytes()

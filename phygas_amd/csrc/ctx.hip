@@ -410,6 +410,10 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         return 1;
     PHX_CK(hipEventRecord(c->ev_fork, c->st));   // uploads, cluster tables and draws first
     for (int i = 1; i < parts; i++) PHX_CK(hipStreamWaitEvent(c->part_stream[i], c->ev_fork, 0));
+    // every sub-pass's head (counters, tracks) is launched before any sub-pass's tail, so that the last
+    // sub-pass's tracks, the start of the pass's longest chain, do not wait for the host to launch the other
+    // sub-passes' fifteen kernels each
+    FastArgs parg[MAX_SPLIT];
     for (int i = 0; i < parts; i++) {
         const int b = c->part_base[i];
         FastArgs ai = a;
@@ -427,8 +431,11 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
             W.scan_spw = c->fw.scan_spw; W.scan_coarse = c->fw.scan_coarse;
             W.use_graphs = c->fw.use_graphs;
         }
-        if (fast_screen(c->fast, W, ai, part_st(c, i), nullptr, nullptr)) return 1;
+        parg[i] = ai;
+        if (fast_screen(c->fast, W, ai, part_st(c, i), nullptr, nullptr, FAST_HEAD)) return 1;
     }
+    for (int i = 0; i < parts; i++)
+        if (fast_screen(c->fast, part_fw(c, i), parg[i], part_st(c, i), nullptr, nullptr, FAST_TAIL)) return 1;
     for (int i = 1; i < parts; i++) {   // the context's stream waits for every part
         PHX_CK(hipEventRecord(c->ev_join[i], c->part_stream[i]));
         PHX_CK(hipStreamWaitEvent(c->st, c->ev_join[i], 0));

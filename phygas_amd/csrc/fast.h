@@ -58,7 +58,13 @@ typedef void (*StageMark)(void *ctx, const char *name);
 
 int fast_build_tables(FastTables &T, const double *d_csoa, const int *d_cn, const double *d_R2, int nc, int npts,
                       hipStream_t st);
-int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx);
+// part: FAST_ALL (the whole pass), FAST_HEAD (counters, draws and tracks) or FAST_TAIL (the rest, after the head
+// on the same stream).  A caller running several passes on several streams launches every head first, so that
+// each pass's first kernels do not wait behind the host's launches of the others' later ones.  A pass captured
+// into a hipGraph is launched whole with its head (its tail is then nothing).
+enum { FAST_ALL = 0, FAST_HEAD = 1, FAST_TAIL = 2 };
+int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx,
+                int part = FAST_ALL);
 // Kept-pair and row counts of the last pass into h[0], h[1]; *grow = 1 when a buffer overflowed (it has
 // been enlarged and the pass must be run again).
 int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow);

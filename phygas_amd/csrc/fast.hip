@@ -1363,14 +1363,15 @@ static int fast_alloc(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t
     return 0;
 }
 
-// The pass's launches only (no allocation, no host synchronisation): capturable into a hipGraph.
-static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
+// The pass's launches only (no allocation, no host synchronisation): capturable into a hipGraph.  The head:
+// counters, draws and tracks.
+static int fast_launch_head(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark,
+                            void *mark_ctx) {
     const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
     unsigned long long *cnt = (unsigned long long *)W.mem[W_CNT];
+    double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, st, cnt, C_TOTAL);
     PHX_CK(hipGetLastError());
-    double *ps0 = (double *)W.mem[W_PS0], *sig = (double *)W.mem[W_SIG], *trk = (double *)W.mem[W_TRK];
-    TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
     if (a.ps0_in) {
         ps0 = const_cast<double *>(a.ps0_in);
         sig = const_cast<double *>(a.sig_in);
@@ -1394,6 +1395,16 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     }
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
+    return 0;
+}
+
+// The rest of the pass, after its head on the same stream: records, scan, point evaluation, finish, rows.
+static int fast_launch_tail(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark,
+                            void *mark_ctx) {
+    const int ns = a.ns, npts = a.npts, ntl = T.ntiles;
+    unsigned long long *cnt = (unsigned long long *)W.mem[W_CNT];
+    double *trk = (double *)W.mem[W_TRK];
+    TileRec1 *srec1 = (TileRec1 *)W.mem[W_SREC1];
     PHX_REQUIRE(npts <= T.npts, "star tracks longer than the cluster tables");
     const double *rot = (const double *)T.mem[T_ROT];
     hipLaunchKernelGGL(k_star_recs<TILE>, dim3(blocks_for((long)ns_pad(ns) * ntl, 256)), dim3(256), 0, st, trk, npts,
@@ -1473,17 +1484,27 @@ static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_
     return 0;
 }
 
+static int fast_launch(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx,
+                       int part = FAST_ALL) {
+    if (part != FAST_TAIL && fast_launch_head(T, W, a, st, mark, mark_ctx)) return 1;
+    if (part != FAST_HEAD && fast_launch_tail(T, W, a, st, mark, mark_ctx)) return 1;
+    return 0;
+}
+
 static void graph_drop(FastWork &W) {
     if (W.gexec) (void)hipGraphExecDestroy((hipGraphExec_t)W.gexec);
     if (W.graph) (void)hipGraphDestroy((hipGraph_t)W.graph);
     W.gexec = W.graph = nullptr;
 }
 
-int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx) {
-    if (fast_alloc(T, W, a, st)) return 1;
+int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, StageMark mark, void *mark_ctx,
+                int part) {
     if (!W.use_graphs || mark) {   // eager launches (timed stages need their events between kernels)
-        return fast_launch(T, W, a, st, mark, mark_ctx);
+        if (part != FAST_TAIL && fast_alloc(T, W, a, st)) return 1;
+        return fast_launch(T, W, a, st, mark, mark_ctx, part);
     }
+    if (part == FAST_TAIL) return 0;   // a captured pass is launched whole with its head
+    if (fast_alloc(T, W, a, st)) return 1;
     // the key of a captured pass: sizes, parameters and every buffer it touches
     uint64_t key = 1469598103934665603ULL;
     auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };

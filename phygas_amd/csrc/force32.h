@@ -73,13 +73,13 @@ struct F3 { float x, y, z; };
 // s^e for s > 0
 PHX_HDF float f_pow(float s, float e) { return f_exp2(e * f_log2(s)); }
 
-// Acceleration at p (FP32).
+// Acceleration at p (FP32), its quotients through the hardware reciprocal (within ~2 ulp).
 PHX_HDF F3 force32(const PotF &Q, float x, float y, float z) {
     const float R2 = f_fma(x, x, y * y), r2 = f_fma(z, z, R2);
     const float u = r2 + Q.b2;
     const float iu = f_rsqrt(u);
     const float fb = -Q.bulge_gm * iu * iu * iu;
-    const float zeta = f_sqrt(f_fma(z, z, Q.db2));
+    const float zeta = f_sqrt_hw(f_fma(z, z, Q.db2));
     const float az = Q.disk_a + zeta;
     const float D2 = f_fma(az, az, R2);
     const float iD = f_rsqrt(D2);
@@ -87,7 +87,7 @@ PHX_HDF F3 force32(const PotF &Q, float x, float y, float z) {
     const float fd = -Q.disk_gm * iD3;
     float fh;
     if (r2 < Q.halo_cut2) {
-        fh = -Q.halo_k1 * f_pow(r2, Q.halo_p1) / (1.0f + Q.halo_k2 * f_pow(r2, Q.halo_p2));
+        fh = f_div(-Q.halo_k1 * f_pow(r2, Q.halo_p1), 1.0f + Q.halo_k2 * f_pow(r2, Q.halo_p2));
     } else {
         const float ir = f_rsqrt(r2);
         fh = -Q.halo_gmcut * ir * ir * ir;
@@ -95,7 +95,7 @@ PHX_HDF F3 force32(const PotF &Q, float x, float y, float z) {
     F3 a;
     a.x = (fb + fd + fh) * x;
     a.y = (fb + fd + fh) * y;
-    a.z = (fb + fh) * z + fd * z * (az / zeta);
+    a.z = (fb + fh) * z + fd * z * f_div(az, zeta);
     return a;
 }
 

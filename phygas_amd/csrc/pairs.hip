@@ -126,8 +126,15 @@ __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 // ---------------------------------------------------------------- traces
 // One clone per lane (reference: src/phygas.c:200-216): target and cluster orbits stepped together, the
 // closest approach on each step's segment, strict-< first minimum.  Fast = 0: the reference's FP64
-// kick-drift-kick (physics.h).  Fast = 1: the FP32 force at the FP64 position with FP64 accumulators and the
-// FP64 segment refinement.
+// kick-drift-kick (physics.h).  Fast = 1: the FP32 force at the FP64 position with FP64 accumulators (each
+// kick and drift one fused multiply-add) and the FP64 segment refinement.
+
+// a * s + c per component, one rounding each
+__device__ __forceinline__ V3 fma3(V3 a, double s, V3 c) {
+    return v3(__builtin_fma(a.x, s, c.x), __builtin_fma(a.y, s, c.y), __builtin_fma(a.z, s, c.z));
+}
+__device__ __forceinline__ V3 dbl3(F3 f) { return v3(f.x, f.y, f.z); }
+
 template <int Fast>
 __device__ __forceinline__ void trace_clone(const double *__restrict__ p, int ns, double dt, const Pot &P,
                                             const PotF &Q, double *__restrict__ R) {
@@ -141,14 +148,14 @@ __device__ __forceinline__ void trace_clone(const double *__restrict__ p, int ns
         vbest = norm(av - bv);
         const double h = 0.5 * dt;
         for (int i = 1; i <= ns; i++) {
-            av = av + v3(fa.x, fa.y, fa.z) * h;
-            bv = bv + v3(fb.x, fb.y, fb.z) * h;
-            ap = ap + av * dt;
-            bp = bp + bv * dt;
+            av = fma3(dbl3(fa), h, av);
+            bv = fma3(dbl3(fb), h, bv);
+            ap = fma3(av, dt, ap);
+            bp = fma3(bv, dt, bp);
             fa = force32(Q, (float)ap.x, (float)ap.y, (float)ap.z);
             fb = force32(Q, (float)bp.x, (float)bp.y, (float)bp.z);
-            av = av + v3(fa.x, fa.y, fa.z) * h;
-            bv = bv + v3(fb.x, fb.y, fb.z) * h;
+            av = fma3(dbl3(fa), h, av);
+            bv = fma3(dbl3(fb), h, bv);
             const V3 r1 = ap - bp;
             const double s = seg_param(r0, r1);
             const double dm = norm(r0 + (r1 - r0) * s);
@@ -215,10 +222,10 @@ __device__ __forceinline__ void trace_clone2(const double *__restrict__ p, int n
     const double h = 0.5 * dt;
     for (int i = 1; i <= ns; i++) {
         if (Fast) {
-            v = v + v3(f.x, f.y, f.z) * h;
-            x = x + v * dt;
+            v = fma3(dbl3(f), h, v);
+            x = fma3(v, dt, x);
             f = force32(Q, (float)x.x, (float)x.y, (float)x.z);
-            v = v + v3(f.x, f.y, f.z) * h;
+            v = fma3(dbl3(f), h, v);
         } else {
             kdk(P, b, dt);
             x = b.pos;

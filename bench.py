@@ -11,10 +11,14 @@ Further sections of the same JSON line:
   two_stage    a 100,000-star shard (seed 4) with its control: the fast screening pass, then the Monte Carlo
                (1,000 clones, device draws, graph replay) of every real row with p_r50 >= 0.01
   serving      the request mix at swept Poisson rates, fast and plain forms (p50/p95/p99, pairs/s)
+  unit_sizes   the fast and the plain form at 5,000, 10,000 and 40,000 stars (+ control), each unit autotuned
   cpu_baseline the reference's own code on the host: 1 thread and N threads, interleaved runs, medians
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fast|plain] [--stars N]
-                    [--no-cpu] [--no-plain] [--no-mc] [--no-two-stage] [--no-serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fast|plain] [--stars N] [--no-tune] [--split N]
+                    [--no-cpu] [--no-plain] [--no-mc] [--no-two-stage] [--no-serve] [--no-sizes]
+
+Before the timed steps the library's autotuner chooses the fast pass's settings on the unit (rows do not depend
+on them); --no-tune times the defaults, --split N fixes the sub-passes and skips the tuner.
 
 Prints one JSON line.
 """
@@ -231,6 +235,35 @@ def monte_carlo(e: hip.Engine, clusters, pulsar, reps: int = 5) -> dict:
     return out
 
 
+def unit_sizes(clusters, no_tune: bool, sizes=(5000, 10000, 40000), steps: int = 5) -> list:
+    """The fast and the plain form at other unit sizes (N stars of seed 2 + their shuffled control x the 4,087
+    clusters): the autotuner run on each unit first (unless --no-tune), then the pass time of each form and the
+    rows of the two compared."""
+    out = []
+    for n in sizes:
+        stars = wl.make_stars(n, 2)
+        control = wl.shuffled_control(stars, 3)
+        with hip.Engine(0) as e:
+            e.load_clusters(clusters, 64, 2.0)
+            e.load_stars(stars, 0)
+            e.load_stars(control, 1)
+            tune = None if no_tune else e.autotune_ex(hip.FAST, 25.0, 64, 0.001)
+            fms, _ = run_form(e, hip.FAST, steps, 1, 25.0, 0.001, stage_passes=0)
+            frows = e.rows()
+            pms, _ = run_form(e, hip.PLAIN, 1, 1, 25.0, 0.001, stage_passes=0)
+            prows = e.rows()
+        a = {(r["star"], r["cluster"]): r["p_r50"] for r in frows}
+        b = {(r["star"], r["cluster"]): r["p_r50"] for r in prows}
+        common = set(a) & set(b)
+        pairs = 2 * n * len(clusters)
+        out.append({"stars": n, "control_stars": n, "pairs": pairs, "fast_ms": fms, "fast_pairs_per_s": pairs / (fms / 1e3),
+                    "plain_ms": pms, "speedup_vs_plain": pms / fms, "rows": len(frows),
+                    "rows_one_sided": len(set(a) ^ set(b)),
+                    "max_abs_dp_r50": max((abs(a[k] - b[k]) for k in common), default=0.0),
+                    "autotune": tune})
+    return out
+
+
 def two_stage(clusters, n_stars: int = 100000, n_clones: int = 1000, p_stage2: float = 0.01) -> dict:
     """A 100,000-star shard (seed 4) with its shuffled control: stage 1 the fast screening pass, stage 2 the
     Monte Carlo of every real row with p_r50 >= 0.01 (device draws, FP32 clones, the batch's launches replayed
@@ -318,10 +351,13 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rates-fast", default="15,60,240,960")
     ap.add_argument("--serve-rates-plain", default="15,30,60")
-    ap.add_argument("--tune", action="store_true", help="run the autotuner (time tile of the scan's first level, "
-                    "stars per task, launch grids) before timing; the library's defaults are its choices on this unit")
+    ap.add_argument("--no-tune", action="store_true", help="time the library's default settings; otherwise the "
+                    "autotuner (sub-passes and the last one's share, the scan's first-level time tile, stars per task, "
+                    "launch grids) runs on the unit before the timed steps, as a long run would")
+    ap.add_argument("--tune", action="store_true", help=argparse.SUPPRESS)   # the default; kept for old commands
     ap.add_argument("--split", type=int, default=0, help="sub-passes of a fast pass on their own streams "
-                    "(0: the library's default, 3)")
+                    "(0: chosen by the autotuner, else fixed and the autotuner not run)")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the fast/plain comparison at other unit sizes")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB device to device (the HBM counter calibration of the profiles) and exit")
     args = ap.parse_args()
@@ -353,7 +389,7 @@ def main() -> int:
 
     form = hip.FAST if args.form == "fast" else hip.PLAIN
     tune = None
-    if form == hip.FAST and args.tune:
+    if form == hip.FAST and not args.no_tune and args.split <= 0:
         tune = e.autotune_ex(hip.FAST, chi2_max, 64, p_min)   # every setting, chosen before the timed steps
     ms, stages = run_form(e, form, args.steps, args.warmup, chi2_max, p_min)
     rows = e.rows()
@@ -400,8 +436,10 @@ def main() -> int:
                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_scan_tiles"),
                     "work_per_launch": work, "avg_launch_ms": k_ms}
 
+    sub_passes = e.tuning()["sub_passes"] if args.split <= 0 else args.split
     mc = None if args.no_mc else monte_carlo(e, clusters, pulsar)
     e.close()
+    sizes = None if args.no_sizes or form != hip.FAST or args.no_plain else unit_sizes(clusters, args.no_tune)
     shard = None if args.no_two_stage else two_stage(clusters)
     serving = None
     if not args.no_serve and args.serve_requests > 0:
@@ -422,13 +460,15 @@ def main() -> int:
         "config": {"workload": "screen_20k_x_4087_with_shuffled_control", "stars": len(stars), "control_stars":
                    len(control), "clusters": len(clusters), "pairs_per_step": pairs, "chi2_max": chi2_max,
                    "n_samp": 64, "r_kernel": 2.0, "p_min": p_min, "form": result["form"],
-                   "sub_passes": args.split if args.split > 0 else 3, "cluster_tables_s": tables_s},
+                   "sub_passes": sub_passes, "cluster_tables_s": tables_s},
         "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "pass_stats": stats, "autotune": tune,
         "roofline": roof, "cpu_baseline": cpu,
     }
     if plain:
         line["plain"] = plain
         line["speedup_vs_plain"] = plain["ms"] / ms
+    if sizes:
+        line["unit_sizes"] = sizes
     if mc:
         line["monte_carlo"] = mc
     if shard:

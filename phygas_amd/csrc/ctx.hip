@@ -76,6 +76,8 @@ struct PhxCtx {
     int last_n_samp = 64;
     FastWork fw;       // fast-form work buffers (part 0 of a split pass)
     int split = 3;     // sub-passes of a large fast pass (phx_set_split)
+    int last_pct = 40; // the last sub-pass's share of the stars, percent (phx_set_tuning)
+    int track_lanes = 0;   // lanes per star of the fast tracks, 0 = chosen per pass (phx_set_track_lanes)
     int nparts = 1;    // sub-passes of the last fast pass, over stars [part_base[i], part_base[i + 1])
     int part_base[MAX_SPLIT + 1] = {};
     long part_rows[MAX_SPLIT] = {};
@@ -169,6 +171,13 @@ extern "C" void *phx_stream(PhxCtx *c) { return (void *)c->st; }
 extern "C" void phx_set_timing(PhxCtx *c, int on) { c->timing = on != 0; }
 
 extern "C" void phx_set_graphs(PhxCtx *c, int on) { c->fw.use_graphs = on != 0; }
+
+extern "C" int phx_set_track_lanes(PhxCtx *c, int lanes) {
+    PHX_REQUIRE(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 3 || lanes == 6,
+                "lanes per star must be 0 (chosen per pass), 1, 2, 3 or 6");
+    c->track_lanes = lanes;
+    return 0;
+}
 
 extern "C" int phx_set_split(PhxCtx *c, int parts) {
     PHX_REQUIRE(parts >= 1 && parts <= MAX_SPLIT, "split must be 1..4");
@@ -390,13 +399,21 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     const int ns = a.ns;
     int parts = c->timing || ns < 2 * SPLIT_MIN ? 1 : std::min(c->split, MAX_SPLIT);
     c->nparts = parts;
-    // the last sub-pass (tracks with one lane per star, the longest) takes 40% of the stars and the others
-    // share the rest equally: measured on the 40,000-star unit, 16.9-17.0 ms per pass against 17.1-17.4 with
-    // equal thirds (the pass time moves with how the sub-passes' kernels meet, not smoothly with the sizes)
-    const int last = parts > 1 ? (int)((long)ns * 2 / 5) : ns;
+    // the last sub-pass (tracks with one lane per star, the longest) takes last_pct of the stars (40% by
+    // default) and the others share the rest equally: measured on the 40,000-star unit, 16.9-17.0 ms per pass
+    // against 17.1-17.4 with equal thirds (the pass time moves with how the sub-passes' kernels meet, not
+    // smoothly with the sizes; the autotuner tries other shares on the caller's batch)
+    const int last = parts > 1 ? (int)((long)ns * c->last_pct / 100) : ns;
     for (int i = 0; i < parts; i++) c->part_base[i] = parts > 1 ? (int)((long)(ns - last) * i / (parts - 1)) : 0;
     c->part_base[parts] = ns;
-    if (parts == 1) return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
+    if (parts == 1) {
+        // a pass run whole: the most lanes per star that still give at most one wavefront per SIMD (the tracks
+        // are a chain of 2,000 sub-steps per lane; more lanes per star shorten it where SIMDs would idle)
+        const long lanes_max = 64L * 4 * c->fast.n_cu;
+        a.track_lanes = c->track_lanes ? c->track_lanes
+                        : (long)ns * 6 <= lanes_max ? 6 : (long)ns * 3 <= lanes_max ? 3 : 2;
+        return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
+    }
     if (!c->ev_fork) PHX_CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     for (int i = 1; i < parts; i++) {
         if (!c->part_stream[i]) PHX_CK(hipStreamCreateWithFlags(&c->part_stream[i], hipStreamNonBlocking));
@@ -423,7 +440,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         ai.sig_in = c->sig.as<double>() + (size_t)b * 36;
         // the last sub-pass's tracks with one lane per star: the sub-passes' tracks kernels then take fewer
         // wavefronts than the device has SIMDs, so none shares a SIMD with another and none waits for one
-        ai.track_lanes = i == parts - 1 ? 1 : 2;
+        ai.track_lanes = c->track_lanes ? c->track_lanes : i == parts - 1 ? 1 : 2;
         FastWork &W = part_fw(c, i);
         if (i > 0) {   // the launch settings of part 0
             W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;
@@ -505,18 +522,22 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
 }
 
 // Launch settings of the fast pass chosen by timing whole passes on the loaded batch as the library runs them
-// (sub-passes overlapping), one knob after another with the others held, best of two passes per option after
-// one not counted: the scan's first level (16-point coarse tiles or the 8-point tiles), its stars per task, and
-// the launch grids of the scan, the item evaluation, the kept-pair reduction and the finish.  Rows do not
-// depend on any of them.  knob_ms[k]: the best pass time while knob k was tuned.
-static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, double knob_ms[6]) {
+// (sub-passes overlapping), one knob after another with the others held, the median of three passes per option
+// after one not counted (overlapping sub-passes vary from pass to pass, so the best of a few would pick luck): the number of sub-passes and the last one's share of the stars, the scan's first level
+// (16-point coarse tiles or the 8-point tiles), its stars per task, and the launch grids of the scan, the item
+// evaluation, the kept-pair reduction and the finish.  Rows do not depend on any of them.  knob_ms[k]: the
+// best pass time while knob k was tuned.
+constexpr int N_KNOBS = 8;
+static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, double knob_ms[N_KNOBS]) {
     PHX_REQUIRE(form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "the autotuner tunes the fast forms");
     PHX_REQUIRE(!c->st_h.empty(), "load stars first");
     if (phx_screen(c, form, chi2_max, n_samp, p_min)) return 1;
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;   // sizes the buffers
     struct Knob { int *v; int opts[5]; };
-    Knob knobs[6] = {{&c->fw.scan_coarse, {1, 0, -1, -1, -1}},
+    Knob knobs[N_KNOBS] = {{&c->split, {3, 2, 4, 1, -1}},
+                     {&c->last_pct, {40, 35, 45, 30, 50}},
+                     {&c->fw.scan_coarse, {1, 0, -1, -1, -1}},
                      {&c->fw.scan_spw, {16, 8, -1, -1, -1}},
                      {&c->fw.scan_blocks, {512, 576, 768, 1024, 2048}},
                      {&c->fw.items_blocks, {4096, 8192, 16384, 32768, 65536}},
@@ -525,19 +546,22 @@ static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, doubl
     const bool timing = c->timing;
     c->timing = false;
     int rc = 0;
-    for (int q = 0; q < 6 && rc == 0; q++) {
+    for (int q = 0; q < N_KNOBS && rc == 0; q++) {
         int best = *knobs[q].v;
         double best_ms = 1e300;
         for (int o = 0; o < 5 && knobs[q].opts[o] >= 0; o++) {
             *knobs[q].v = knobs[q].opts[o];
-            double t = 1e300;
-            for (int rep = 0; rep < 3 && rc == 0; rep++) {   // the first pass with new settings is not counted
+            double runs[3];
+            for (int rep = 0; rep < 4 && rc == 0; rep++) {   // the first pass with new settings is not counted
                 const auto t0 = std::chrono::steady_clock::now();
                 if ((rc = phx_screen(c, form, chi2_max, n_samp, p_min)) != 0) break;
                 if (hipStreamSynchronize(c->st) != hipSuccess) { rc = 1; break; }
                 const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                if (rep > 0) t = std::min(t, ms);
+                if (rep > 0) runs[rep - 1] = ms;
             }
+            if (rc) break;
+            std::sort(runs, runs + 3);
+            const double t = runs[1];
             if (rc == 0 && t < best_ms) { best_ms = t; best = knobs[q].opts[o]; }
         }
         *knobs[q].v = best;
@@ -548,12 +572,12 @@ static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, doubl
 }
 
 extern "C" int phx_autotune(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, int *grids, double *ms) {
-    double kms[6];
+    double kms[N_KNOBS];
     if (int rc = autotune_fast(c, form, chi2_max, n_samp, p_min, kms)) return rc;
     const int g[4] = {c->fw.scan_blocks, c->fw.items_blocks, c->fw.reduce_blocks, c->fw.finish_blocks};
     for (int i = 0; i < 4; i++) {
         if (grids) grids[i] = g[i];
-        if (ms) ms[i] = kms[2 + i];
+        if (ms) ms[i] = kms[N_KNOBS - 4 + i];
     }
     return 0;
 }
@@ -575,16 +599,18 @@ extern "C" int phx_get_tuning(PhxCtx *c, PhxTuning *t) {
     t->finish_blocks = c->fw.finish_blocks;
     t->stars_per_task = c->fw.scan_spw;
     t->coarse_tiles = c->fw.scan_coarse;
+    t->sub_passes = c->split;
+    t->last_share_pct = c->last_pct;
     t->pass_ms = 0;
     return 0;
 }
 
 extern "C" int phx_autotune_ex(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, PhxTuning *out) {
-    double kms[6];
+    double kms[N_KNOBS];
     if (int rc = autotune_fast(c, form, chi2_max, n_samp, p_min, kms)) return rc;
     if (out) {
         phx_get_tuning(c, out);
-        out->pass_ms = kms[5];
+        out->pass_ms = kms[N_KNOBS - 1];
     }
     return 0;
 }
@@ -593,10 +619,15 @@ extern "C" int phx_set_tuning(PhxCtx *c, const PhxTuning *t) {
     PHX_REQUIRE(t != nullptr, "tuning is null");
     PHX_REQUIRE(t->stars_per_task == 8 || t->stars_per_task == 16, "stars per task must be 8 or 16");
     PHX_REQUIRE(t->coarse_tiles == 0 || t->coarse_tiles == 1, "coarse_tiles must be 0 or 1");
+    PHX_REQUIRE(t->sub_passes <= MAX_SPLIT, "sub_passes must be 1..4 (0 keeps the current setting)");
+    PHX_REQUIRE(t->last_share_pct <= 0 || (t->last_share_pct >= 20 && t->last_share_pct <= 60),
+                "last_share_pct must be 20..60 (0 keeps the current setting)");
     const int g[4] = {t->scan_blocks, t->items_blocks, t->reduce_blocks, t->finish_blocks};
     if (int rc = phx_set_grids(c, g)) return rc;
     c->fw.scan_spw = t->stars_per_task;
     c->fw.scan_coarse = t->coarse_tiles;
+    if (t->sub_passes > 0) c->split = t->sub_passes;
+    if (t->last_share_pct > 0) c->last_pct = t->last_share_pct;
     return 0;
 }
 

@@ -140,20 +140,24 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
 }
 
 // ------------------------------------------------------------------------------------------- tracks
-// LPS lanes per star.  LPS = 2: lane q propagates offsets 3q .. 3q+2 (x, y, z for q = 0; vx, vy, vz for
-// q = 1) and both lanes run the same FP64 nominal.  LPS = 1: one lane propagates all six offsets and runs
-// the nominal once (fewer operations per star, a longer chain per lane).  Both give the same bits: every
-// offset's arithmetic is the same.  Track layout as the plain path: [star][npts][12] FP64.
+// LPS lanes per star (1, 2, 3 or 6): lane q of a star propagates offsets NO q .. NO q + NO - 1 (NO = 6 / LPS;
+// offset o < 3 is position component o, o >= 3 velocity component o - 3) and every lane of the star runs the
+// same FP64 nominal.  Fewer lanes per star: fewer operations per star, a longer chain per lane; more lanes: the
+// other way round, for batches that would leave SIMDs idle.  All give the same bits: every offset's arithmetic
+// is the same.  A wavefront holds 64 / LPS whole stars (lanes past them idle), so the gather of the six
+// columns by lane 0 of a star stays inside the wavefront.  Track layout as the plain path: [star][npts][12] FP64.
 template <int LPS>
 __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
                                                   double *__restrict__ trk, Pot P, PotF Q, int npts, int n,
                                                   double *__restrict__ tiled, int ntl) {
 #pragma clang fp contract(fast)
-    static_assert(LPS == 1 || LPS == 2, "one or two lanes per star");
-    constexpr int NO = 6 / LPS;   // offsets per lane
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = gid / LPS, q = LPS == 2 ? (gid & 1) : 0;
-    const bool live = s < n;
+    static_assert(LPS == 1 || LPS == 2 || LPS == 3 || LPS == 6, "1, 2, 3 or 6 lanes per star");
+    constexpr int NO = 6 / LPS;     // offsets per lane
+    constexpr int SPW = 64 / LPS;   // stars per wavefront
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+    const int s = (gid >> 6) * SPW + lane / LPS, q = lane % LPS;
+    const bool live = lane < SPW * LPS && s < n;
+    const int base = lane - q;      // lane 0 of the star
     const double eps = PH_LIN_EPS_POS, inv_eps = 1.0 / PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
     V3 x = v3(0, 0, 0), v = v3(0, 0, 0);
     if (live) {
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
     for (int i = 0; i < NO; i++)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const int o = 3 * q + i;   // offset o: position component o (o < 3), velocity component o - 3
+            const int o = NO * q + i;
             dx[i][c] = (o < 3 && c == o) ? eps : 0.0;
             dv[i][c] = (o >= 3 && c == o - 3) ? eps : 0.0;
         }
@@ -212,24 +216,15 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
                 dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
             }
         }
-        // A[r][o] = dx[o][r] / eps; with two lanes, lane 0 receives lane 1's three columns
+        // A[r][o] = dx[o][r] / eps: column o from lane o / NO of the star, its offset o % NO
         double A[3][6];
-        if (LPS == 2) {
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+        for (int o = 0; o < 6; o++)
 #pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    const double mine = dx[i % NO][r] * inv_eps;
-                    const double other = __shfl_xor(mine, 1, 64);
-                    A[r][i] = q == 0 ? mine : other;
-                    A[r][3 + i] = q == 0 ? other : mine;
-                }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 6; i++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) A[r][i] = dx[i % NO][r] * inv_eps;
-        }
+            for (int r = 0; r < 3; r++) {
+                const double mine = dx[o % NO][r] * inv_eps;
+                A[r][o] = LPS == 1 ? mine : __shfl(mine, base + o / NO, 64);
+            }
         if (!live || q != 0) continue;
         const double *sg = sig + (size_t)s * 36;
         double AS[3][6];
@@ -1386,12 +1381,12 @@ static int fast_launch_head(FastTables &T, FastWork &W, const FastArgs &a, hipSt
         hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts,
                            ns, npts, ntl, (double *)W.mem[W_TILED]);
     } else {   // writes the tiled copy too
-        if (a.track_lanes == 1)
-            hipLaunchKernelGGL(k_lin_fast<1>, dim3(blocks_for((long)ns, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
-                               make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
-        else
-            hipLaunchKernelGGL(k_lin_fast<2>, dim3(blocks_for((long)ns * 2, 256)), dim3(256), 0, st, ps0, sig, trk,
-                               a.P, make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
+        const int lps = a.track_lanes;
+        PHX_REQUIRE(lps == 1 || lps == 2 || lps == 3 || lps == 6, "lanes per star must be 1, 2, 3 or 6");
+        auto kern = lps == 1 ? k_lin_fast<1> : lps == 2 ? k_lin_fast<2> : lps == 3 ? k_lin_fast<3> : k_lin_fast<6>;
+        const long waves = ((long)ns + 64 / lps - 1) / (64 / lps);   // whole stars per wavefront
+        hipLaunchKernelGGL(kern, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
+                           make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
     }
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");

@@ -243,6 +243,7 @@ def test_autotune_keeps_rows_and_stats_consistent(unit):
         _report("autotune_stars_per_task", tune["stars_per_task"])
         _report("autotune_coarse_tiles", tune["coarse_tiles"])
         assert tune["stars_per_task"] in (8, 16) and tune["coarse_tiles"] in (0, 1) and tune["pass_ms"] > 0
+        assert 1 <= tune["sub_passes"] <= 4 and 30 <= tune["last_share_pct"] <= 50
         seen = {}
         for coarse in (1, 0):
             for spw in (16, 8):
@@ -289,8 +290,8 @@ def test_fast64_edge_batches_identical_to_plain(workload, ns):
 
 
 def test_split_passes_give_the_same_rows(workload):
-    """A pass split into 2, 3 or 4 sub-passes on their own streams gives the whole pass's rows, work counts and
-    tracks (also with hipGraph replays)."""
+    """A pass split into 2, 3 or 4 sub-passes on their own streams, with the last one's share of the stars from 30
+    to 60%, gives the whole pass's rows, work counts and tracks (also with hipGraph replays)."""
     cl = workload.make_clusters(4087, 0)
     st = workload.make_stars(2500, 2)
     ctl = workload.shuffled_control(st, 3)
@@ -307,6 +308,11 @@ def test_split_passes_give_the_same_rows(workload):
                 rows = [e.screen(hip.FAST, chi2_max=25.0, p_min=p) for p in (0.0, 0.001)]
                 out[(parts, graphs)] = (rows, e.pass_stats(), e.star_tracks(1240, 20))
         e.set_graphs(False)
+        for parts, share in ((3, 30), (3, 50), (2, 60)):   # the last sub-pass's share of the stars
+            e.set_tuning(sub_passes=parts, last_share_pct=share)
+            rows = [e.screen(hip.FAST, chi2_max=25.0, p_min=p) for p in (0.0, 0.001)]
+            out[(parts, share)] = (rows, e.pass_stats(), e.star_tracks(1240, 20))
+        e.set_tuning(sub_passes=3, last_share_pct=40)
     ref_rows, ref_stats, ref_trk = out[(1, False)]
     assert len(ref_rows[0]) > 1000 and len(ref_rows[1]) > 50
     assert ref_trk.shape == (20, npts, 12)
@@ -374,3 +380,27 @@ def test_device_draws_match_host_reference(workload, oracle):
         assert (np.abs(fast[f] - want[f]) / np.maximum(np.abs(want[f]), 1e-6)).max() < 1e-3, f
     _report("device_draws_quantile_rel_max", worst)
     assert np.array_equal(got["f_r50"], want["f_r50"])
+
+
+def test_track_lanes_give_the_same_bits(workload):
+    """The fast tracks with 1, 2, 3 or 6 lanes per star (the library's choice for a whole pass of this size: 6):
+    tracks, rows and work counts bit for bit the same.  171 stars + control leave the last wavefront of every lane
+    count partly empty."""
+    cl = workload.make_clusters(4087, 0)
+    st = workload.make_stars(171, 2)
+    ctl = workload.shuffled_control(st, 3)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        n = len(st) + len(ctl)
+        ref = None
+        for lanes in (0, 1, 2, 3, 6):
+            e.set_track_lanes(lanes)
+            rows = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
+            got = (rows.tobytes(), e.pass_stats(), e.star_tracks(0, n).tobytes())
+            if ref is None:
+                ref = got
+                assert len(rows) > 100
+            assert got == ref, lanes
+        e.set_track_lanes(0)

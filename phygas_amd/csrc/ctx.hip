@@ -522,10 +522,11 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
 }
 
 // Launch settings of the fast pass chosen by timing whole passes on the loaded batch as the library runs them
-// (sub-passes overlapping), one knob after another with the others held, the median of three passes per option
-// after one not counted (overlapping sub-passes vary from pass to pass, so the best of a few would pick luck): the number of sub-passes and the last one's share of the stars, the scan's first level
-// (16-point coarse tiles or the 8-point tiles), its stars per task, and the launch grids of the scan, the item
-// evaluation, the kept-pair reduction and the finish.  Rows do not depend on any of them.  knob_ms[k]: the
+// (sub-passes overlapping), one knob after another with the others held, the mean of five passes per option
+// after one not counted, each pass checked complete (overlapping sub-passes vary from pass to pass: the best or
+// the median of a few would pick luck; throughput is the mean): the number of sub-passes and the last one's
+// share of the stars, the scan's first level (16-point coarse tiles or the 8-point tiles), its stars per task,
+// and the launch grids of the scan, the item evaluation, the kept-pair reduction and the finish.  Rows do not depend on any of them.  knob_ms[k]: the
 // best pass time while knob k was tuned.
 constexpr int N_KNOBS = 8;
 static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, double knob_ms[N_KNOBS]) {
@@ -551,17 +552,20 @@ static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, doubl
         double best_ms = 1e300;
         for (int o = 0; o < 5 && knobs[q].opts[o] >= 0; o++) {
             *knobs[q].v = knobs[q].opts[o];
-            double runs[3];
-            for (int rep = 0; rep < 4 && rc == 0; rep++) {   // the first pass with new settings is not counted
+            constexpr int NRUN = 5;
+            double runs[NRUN];
+            for (int rep = 0; rep <= NRUN && rc == 0; rep++) {   // the first pass with new settings is not counted
                 const auto t0 = std::chrono::steady_clock::now();
                 if ((rc = phx_screen(c, form, chi2_max, n_samp, p_min)) != 0) break;
-                if (hipStreamSynchronize(c->st) != hipSuccess) { rc = 1; break; }
+                // complete passes only: a setting whose buffers overflow is run again with larger ones (the
+                // first, uncounted pass of an option grows them), as a caller's pass would be
+                if ((rc = last_counts(c, &kept, &nr)) != 0) break;
                 const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 if (rep > 0) runs[rep - 1] = ms;
             }
             if (rc) break;
-            std::sort(runs, runs + 3);
-            const double t = runs[1];
+            double t = 0.0;
+            for (double r : runs) t += r / NRUN;
             if (rc == 0 && t < best_ms) { best_ms = t; best = knobs[q].opts[o]; }
         }
         *knobs[q].v = best;

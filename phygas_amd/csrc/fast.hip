@@ -63,13 +63,15 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_ITEMS, W_IRES, W_ISTART,
        W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SRECFC, W_SRECF, W_NSLOT };
 static_assert(W_NSLOT <= 24, "FastWork holds 24 buffers");
-// Counters (W_CNT): candidates, kept pairs, rows, items, scan task, level-2 tests of the scan, draws left
+// Counters (W_CNT): candidates, kept pairs, rows, items, (unused), level-2 tests of the scan, draws left
 // for the exact evaluation by the finish screen, largest kept segment, level-1 tests of the scan
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_TASK, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_UNUSED4, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
 // The reduce appends kept pairs to KSEG segments of the W_KSEG list, each with its own counter (counter i at
 // C_KSEG0 + i KSEG_STRIDE, one per 256 bytes): one counter for every wavefront of the grid would take its
 // atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
-constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32, C_TOTAL = C_KSEG0 + KSEG * KSEG_STRIDE;
+constexpr int KSEG = 64, KSEG_STRIDE = 32, C_KSEG0 = 32;
+// The scan's task queues, one per XCD (counter x at C_TASKQ0 + x KSEG_STRIDE).
+constexpr int N_XCD = 8, C_TASKQ0 = C_KSEG0 + KSEG * KSEG_STRIDE, C_TOTAL = C_TASKQ0 + N_XCD * KSEG_STRIDE;
 // Table buffers (FastTables::mem slots)
 enum { T_CTRK, T_CTILED, T_CREC1, T_CNT, T_CPERM, T_CNS, T_CREC1C, T_CRECFC, T_CRECF, T_ROT, T_NSLOT };
 static_assert(T_NSLOT <= 16, "FastTables holds 16 buffers");
@@ -324,8 +326,13 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
 }
 
 // ------------------------------------------------------------------------------------------- tile scan
-// Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) from a
-// counter until none is left, lane = cluster.  Per coarse tile J (TILE2 = 2 TILE points: tiles 2J, 2J + 1):
+// Persistent: each wavefront takes tasks (block of SPW stars x block of 64 clusters sorted by length) until
+// none is left, lane = cluster.  The tasks are split over N_XCD queues by cluster block (queue x: the blocks
+// cb = x mod N_XCD, star-block-major), and a workgroup starts on the queue of its XCD (workgroups go to the
+// XCDs in turn): each XCD's L2 then holds the records of its eighth of the cluster blocks instead of all of
+// them, and the star blocks pass through it in order.  A wavefront whose queue is empty takes tasks from the
+// next queues, so every queue drains and every wavefront reaches the exit.  The order of the tasks changes
+// nothing but the order of the candidate list.  Per coarse tile J (TILE2 = 2 TILE points: tiles 2J, 2J + 1):
 //   level 1  every star of the block against the lane's cluster, on the coarse records (the star's record
 //            is wave-uniform and read through scalar loads);
 //   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     const TileRecF *__restrict__ srecf, int ns, int ntl, int ntlc,
                                                     float kchi, double chi2c, Cand *__restrict__ cand,
                                                     unsigned long long *__restrict__ ncand, long cap,
-                                                    unsigned long long *__restrict__ next_task,
+                                                    unsigned long long *__restrict__ task_q,
                                                     unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
     static_assert(SPWT == 8 || SPWT == 16, "8 or 16 stars per task");
@@ -384,7 +391,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     unsigned short *q = q_s[w];
     float (*c2s)[64] = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPWT - 1) / SPWT;
-    const int ntasks = ncb * nsb;
+    const int xcd = blockIdx.x % N_XCD;
+    int qi = 0;   // queues tried: own, then the following ones (wave-uniform)
     // exclusive prefix of a per-lane count over the wavefront; the total into *tot
     auto excl_prefix = [&](unsigned cnt, int *tot) {
         unsigned x = cnt;
@@ -404,12 +412,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     };
     for (;;) {
-        // tasks in star-block-major order, taken dynamically: the waves in flight share few star blocks
-        int t = 0;
-        if (lane == 0) t = (int)atomicAdd(next_task, 1ull);
-        const int task = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
-        if (task >= ntasks) break;
-        const int sb = task / ncb, cb = task % ncb;
+        int sb = -1, cb = 0;
+        while (qi < N_XCD) {   // wave-uniform
+            const int qx = (xcd + qi) % N_XCD;
+            const int nbq = qx < ncb ? (ncb - qx + N_XCD - 1) / N_XCD : 0;   // cluster blocks of queue qx
+            int t = 0;
+            if (lane == 0) {
+                const unsigned long long v = atomicAdd(task_q + (size_t)qx * KSEG_STRIDE, 1ull);
+                t = v < 0x7fffffffull ? (int)v : 0x7fffffff;
+            }
+            const int task = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+            if (task < nsb * nbq) {   // star-block-major within the queue
+                sb = task / nbq;
+                cb = qx + N_XCD * (task % nbq);
+                break;
+            }
+            qi++;
+        }
+        if (sb < 0) break;
         const int cs = cb * 64 + lane;
         const int nt = cs < nc ? ntc_sorted[cs] : 0;
         const int ncl = cs < nc ? cn_sorted[cs] : 0;
@@ -1427,7 +1447,7 @@ static int fast_launch_tail(FastTables &T, FastWork &W, const FastArgs &a, hipSt
                        (const int *)T.mem[T_CPERM], T.nc, T.nc_pad, (const TileRec1 *)W.mem[W_SREC1C],
                        (const TileRecF *)W.mem[W_SRECFC], (const TileRecF *)W.mem[W_SRECF], ns, ntl, n_coarse(ntl),
                        up(sqrt(chi2c)), chi2c,
-                       (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASK, cnt + C_L1, cnt + C_L2);
+                       (Cand *)W.mem[W_CAND], cnt + C_CAND, W.cand_cap, cnt + C_TASKQ0, cnt + C_L1, cnt + C_L2);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "scan");
     const Cand *cand = (const Cand *)W.mem[W_CAND];

@@ -161,9 +161,12 @@ __global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0
     const bool live = lane < SPW * LPS && s < n;
     const int base = lane - q;      // lane 0 of the star
     const double eps = PH_LIN_EPS_POS, inv_eps = 1.0 / PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
+    // lanes without a star of their own (past the wavefront's whole stars or the batch's end) integrate a real
+    // star's orbit without writing it: a lane left at the origin would fail the step's range check every
+    // sub-step and hold the whole wavefront in the branching form of the difference force
     V3 x = v3(0, 0, 0), v = v3(0, 0, 0);
-    if (live) {
-        const double *p = ps0 + (size_t)s * 6;
+    if (n > 0) {
+        const double *p = ps0 + (size_t)min(s, n - 1) * 6;
         x = v3(p[0], p[1], p[2]);
         v = v3(p[3], p[4], p[5]);
     }

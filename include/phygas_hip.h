@@ -111,9 +111,9 @@ int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
  * No counterpart in the reference (its screen is one host loop over the targets, src/phygas_screen.c:48-67). */
 int phx_set_split(PhxCtx *ctx, int parts);
 /* Lanes per star of the fast tracks kernel: 1, 2, 3 or 6 for every (sub-)pass, or 0 (default): a pass run whole
- * takes the most lanes (6, 3, then 2) that still give at most one wavefront per SIMD, a split pass two lanes per
- * star in its sub-passes and one in the last.  Tracks and rows do not depend on it.  No counterpart in the
- * reference. */
+ * takes six lanes per star while that gives at most one wavefront per SIMD, else three while that fits two
+ * wavefronts per SIMD, else two; a split pass two lanes per star in its sub-passes and one in the last.  Tracks
+ * and rows do not depend on it.  No counterpart in the reference. */
 int phx_set_track_lanes(PhxCtx *ctx, int lanes);
 
 /* Launch grids of the fast pass's tile scan, item evaluation, kept-pair reduction and finish, chosen one

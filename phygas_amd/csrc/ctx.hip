@@ -407,11 +407,14 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     for (int i = 0; i < parts; i++) c->part_base[i] = parts > 1 ? (int)((long)(ns - last) * i / (parts - 1)) : 0;
     c->part_base[parts] = ns;
     if (parts == 1) {
-        // a pass run whole: the most lanes per star that still give at most one wavefront per SIMD (the tracks
-        // are a chain of 2,000 sub-steps per lane; more lanes per star shorten it where SIMDs would idle)
+        // a pass run whole: six lanes per star while that gives at most one wavefront per SIMD, else three
+        // while that fits the two wavefronts per SIMD their registers allow, else two (the tracks are a chain
+        // of 2,000 sub-steps per lane; more lanes per star shorten it where SIMDs would idle).  Measured tracks
+        // of whole passes: 10,000 stars 1.66 ms with six lanes (1.99 with three, 2.45 with two); 20,000 stars
+        // 2.56 with three (3.21 with six, 3.07 with two); 40,000 stars 4.26 with three (5.27 with two)
         const long lanes_max = 64L * 4 * c->fast.n_cu;
         a.track_lanes = c->track_lanes ? c->track_lanes
-                        : (long)ns * 6 <= lanes_max ? 6 : (long)ns * 3 <= lanes_max ? 3 : 2;
+                        : (long)ns * 6 <= lanes_max ? 6 : (long)ns * 3 <= 2 * lanes_max ? 3 : 2;
         return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
     }
     if (!c->ev_fork) PHX_CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));

@@ -88,7 +88,7 @@ def _kv(line: str) -> dict:
 
 def cpu_baseline(clusters, stars, pulsar, threads: int, reps: int = 3) -> dict | None:
     """The reference's own code (oracle/_ref, built from the reference sources) or, without it, the oracle
-    restatement, on the GPU box's host cores: the screening loop of phygas_screen on a bounded sample (25 stars
+    restatement, on the host cores of the machine the bench runs on: the screening loop of phygas_screen on a bounded sample (25 stars
     per thread against every cluster, the chi2 cut and the finish) and ph_pair (5 pairs of 200 clones per
     thread), each at 1 thread and at `threads` threads, interleaved over `reps` rounds, medians reported."""
     exe, kind = _cpu_exe()

@@ -4,10 +4,9 @@
   byte: this pins the exactness of the tile/point culling and of the draw screening of the finish.
 * FAST (FP32-offset tracks) against the oracle: positions within 1e-9 relative, covariances within 2e-5
   (relative to sqrt(S_ii S_jj)), chi2 within 1e-4 relative; p_r50 / p_rtot counts identical for every kept
-  pair on these inputs (the north-star bar is |dp_r50| <= 1e-4, i.e. the same count); kept sets equal except
-  pairs within 1e-4 of the chi2 cut.  Measured errors go to gpurun_out/parity_fast.json.
+  pair on these inputs (the bar is |dp_r50| <= 1e-4, i.e. the same count); kept sets equal except
+  pairs within 1e-4 of the chi2 cut.  Measured errors go to parity_fast.json in the report directory.
 """
-import json
 import os
 
 import numpy as np
@@ -15,6 +14,8 @@ import pytest
 
 from phygas_amd import hip
 from phygas_amd import io as pio
+
+import parity_report
 
 pytestmark = pytest.mark.gpu
 
@@ -25,9 +26,7 @@ REPORT = {}
 
 def _report(key, value):
     REPORT[key] = float(value)
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "parity_fast.json"), "w") as f:
-        json.dump(REPORT, f, indent=1)
+    parity_report.write("parity_fast.json", REPORT)
 
 
 @pytest.fixture(scope="module")
@@ -338,7 +337,7 @@ def pulsar_oracle(pulsar_unit, oracle):
 
 @pytest.mark.parametrize("form", ["plain", "fast"])
 def test_pair_forms_match_oracle(pulsar_unit, pulsar_oracle, form):
-    """Monte Carlo quantiles within 1e-3 relative and identical clone counts; the plain form
+    """Monte Carlo quantiles within 1e-3 relative (the Monte Carlo bar) and identical clone counts; the plain form
     within 1e-9.  Fractions may differ only by clones within rounding of r50 / rtot."""
     t, cl = pulsar_unit
     ti, ci, want = pulsar_oracle

@@ -9,9 +9,8 @@
 * The two-stage shard on a reduced sample (600 stars of seed 4): stage 1 screens, stage 2 runs the exact Monte
   Carlo with 1,000 clones per pair for every row with p_r50 >= 0.01; against the oracle's ph_pair on the same
   pairs.  Bar: quantiles within 1e-3 relative (the Monte Carlo bar; the plain form's measured maxima go to
-  gpurun_out/parity_pairs.json), f_r50 / f_rtot identical.
+  parity_pairs.json in the report directory), f_r50 / f_rtot identical.
 """
-import json
 import os
 
 import numpy as np
@@ -19,6 +18,8 @@ import pytest
 
 from phygas_amd import hip
 from phygas_amd import io as pio
+
+import parity_report
 
 pytestmark = pytest.mark.gpu
 
@@ -30,9 +31,7 @@ REPORT = {}
 
 def _report(key, value):
     REPORT[key] = float(value)
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "parity_pairs.json"), "w") as f:
-        json.dump(REPORT, f, indent=1)
+    parity_report.write("parity_pairs.json", REPORT)
 
 
 def _qrel(a, b):

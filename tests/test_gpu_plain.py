@@ -7,8 +7,7 @@ rounded (crmath.h) and every object's nominal start state from the host's C libr
 same doubles as the oracle except where glibc itself misrounds (about 0.1% of pow calls); the finite-
 difference state-transition matrix, A = (x_j - x_0) / 1e-5, amplifies such a difference 1e5-fold, which is why
 this is a tolerance and not bit identity (tests/test_libm_floor.py measures that floor on the CPU).  The
-measured maxima and the share of bit-identical values go to gpurun_out/parity_plain.json."""
-import json
+measured maxima and the share of bit-identical values go to parity_plain.json in the report directory."""
 import os
 
 import numpy as np
@@ -17,6 +16,8 @@ import pytest
 from phygas_amd import hip
 from phygas_amd import io as pio
 from phygas_amd import rows as prows
+
+import parity_report
 
 pytestmark = pytest.mark.gpu
 
@@ -28,9 +29,7 @@ REPORT = {}
 
 def _report(key, value):
     REPORT[key] = max(float(value), REPORT.get(key, 0.0))
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "parity_plain.json"), "w") as f:
-        json.dump(REPORT, f, indent=1)
+    parity_report.write("parity_plain.json", REPORT)
 
 
 @pytest.fixture(scope="module")

@@ -13,14 +13,15 @@ errors are relative to max(chi2, 1).
 * mode 3, the FP32 nominal without compensation: positions off by 1e-5, chi2 by 1e-3.
 * mode 1, all seven orbits in FP32 with compensation, the matrix by differencing them: the 1e-5 offsets lose
   their FP32 force difference (covariances wrong by their own size), so the difference force is needed.
-Measured figures go to gpurun_out/precision_modes.json."""
+Measured figures go to precision_modes.json in the report directory."""
 import ctypes as C
-import json
 import os
 import subprocess
 
 import numpy as np
 import pytest
+
+import parity_report
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -143,9 +144,7 @@ def _measure(harness, oracle, setup, ref_rows, mode):
              dp_rtot=max(abs(rows[k][2] - ref_rows[k][2]) for k in both),
              kb_differs=sum(rows[k][3] != ref_rows[k][3] for k in both))
     REPORT[f"mode{mode}"] = m
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "precision_modes.json"), "w") as f:
-        json.dump(REPORT, f, indent=1)
+    parity_report.write("precision_modes.json", REPORT)
     return got, m
 
 

@@ -352,8 +352,9 @@ def main() -> int:
     ap.add_argument("--serve-rates-fast", default="15,60,240,960")
     ap.add_argument("--serve-rates-plain", default="15,30,60")
     ap.add_argument("--no-tune", action="store_true", help="time the library's default settings; otherwise the "
-                    "autotuner (sub-passes and the last one's share, the scan's first-level time tile, stars per task, "
-                    "launch grids) runs on the unit before the timed steps, as a long run would")
+                    "autotuner (sub-passes and the last one's share, the tracks' offsets per lane, the scan's "
+                    "first-level time tile, stars per task, launch grids) runs on the unit before the timed steps, "
+                    "as a long run would")
     ap.add_argument("--tune", action="store_true", help=argparse.SUPPRESS)   # the default; kept for old commands
     ap.add_argument("--split", type=int, default=0, help="sub-passes of a fast pass on their own streams "
                     "(0: chosen by the autotuner, else fixed and the autotuner not run)")

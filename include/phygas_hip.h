@@ -106,15 +106,15 @@ void phx_set_timing(PhxCtx *ctx, int on);
 void phx_set_graphs(PhxCtx *ctx, int on);
 int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
 /* A fast pass over a batch of at least 4,096 stars runs as `parts` sub-passes (1..4; default 3) over
- * contiguous star ranges, each on its own stream, so that one's latency-bound tracks overlap another's scan
- * (the last sub-pass's tracks take one lane per star); rows are the same.  A pass with timing on runs whole.
+ * contiguous star ranges, each on its own stream, so that one's latency-bound tracks overlap another's scan;
+ * rows are the same.  A pass with timing on runs whole.
  * No counterpart in the reference (its screen is one host loop over the targets, src/phygas_screen.c:48-67). */
 int phx_set_split(PhxCtx *ctx, int parts);
-/* Lanes per star of the fast tracks kernel: 1, 2, 3 or 6 for every (sub-)pass, or 0 (default): a pass run whole
- * takes six lanes per star while that gives at most one wavefront per SIMD, else three while that fits two
- * wavefronts per SIMD, else two; a split pass two lanes per star in its sub-passes and one in the last.  Tracks
- * and rows do not depend on it.  No counterpart in the reference. */
-int phx_set_track_lanes(PhxCtx *ctx, int lanes);
+/* Offsets per lane of the fast tracks kernel (1, 2 or 3; 0 restores the default, 2).  The kernel runs one
+ * workgroup per 64 stars: one wavefront integrates their FP64 nominal orbits, the others their six
+ * finite-difference offsets, this many per lane.  Tracks and rows do not depend on it.  No counterpart in the
+ * reference. */
+int phx_set_track_offsets(PhxCtx *ctx, int offsets);
 
 /* Launch grids of the fast pass's tile scan, item evaluation, kept-pair reduction and finish, chosen one
  * after another by timing whole passes over the loaded batch as the library runs them (results do not
@@ -130,18 +130,20 @@ int phx_set_grids(PhxCtx *ctx, const int *grids);
  * wavefront over 64 clusters of similar track length, the clusters sorted by length into buckets of 64);
  * coarse_tiles: the scan's first level on 16-point time tiles, then their 8-point tiles (1, default), or on
  * the 8-point tiles directly (0); sub_passes: as phx_set_split (1..4, default 3); last_share_pct: the last
- * sub-pass's share of the stars (20..60 %, default 40).  In phx_set_tuning sub_passes and last_share_pct <= 0
- * keep the current setting.  Rows do not depend on any setting.  No counterpart in the reference. */
+ * sub-pass's share of the stars (20..60 %, default 40); track_offsets: as phx_set_track_offsets (1..3, default
+ * 2).  In phx_set_tuning sub_passes, last_share_pct and track_offsets <= 0 keep the current setting.  Rows do
+ * not depend on any setting.  No counterpart in the reference. */
 typedef struct {
     int scan_blocks, items_blocks, reduce_blocks, finish_blocks;   /* launch grids; <= 0: the library's default */
     int stars_per_task;
     int coarse_tiles;
     int sub_passes;
     int last_share_pct;
+    int track_offsets;
     double pass_ms;   /* phx_autotune_ex: the pass time with the settings chosen */
 } PhxTuning;
-/* phx_autotune over every setting (sub-passes and the last one's share, the time tile of the scan's first level,
- * stars per task, the four grids); the choices stay set on the context and are returned in *out. */
+/* phx_autotune over every setting (sub-passes and the last one's share, the tracks' offsets per lane, the time
+ * tile of the scan's first level, stars per task, the four grids); the choices stay set on the context and are returned in *out. */
 int phx_autotune_ex(PhxCtx *ctx, int form, double chi2_max, int n_samp, double p_min, PhxTuning *out);
 int phx_set_tuning(PhxCtx *ctx, const PhxTuning *t);
 int phx_get_tuning(PhxCtx *ctx, PhxTuning *t);

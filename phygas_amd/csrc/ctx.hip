@@ -77,7 +77,7 @@ struct PhxCtx {
     FastWork fw;       // fast-form work buffers (part 0 of a split pass)
     int split = 3;     // sub-passes of a large fast pass (phx_set_split)
     int last_pct = 40; // the last sub-pass's share of the stars, percent (phx_set_tuning)
-    int track_lanes = 0;   // lanes per star of the fast tracks, 0 = chosen per pass (phx_set_track_lanes)
+    int track_offsets = 2; // offsets per lane of the fast tracks' offset wavefronts (phx_set_track_offsets)
     int nparts = 1;    // sub-passes of the last fast pass, over stars [part_base[i], part_base[i + 1])
     int part_base[MAX_SPLIT + 1] = {};
     long part_rows[MAX_SPLIT] = {};
@@ -172,10 +172,9 @@ extern "C" void phx_set_timing(PhxCtx *c, int on) { c->timing = on != 0; }
 
 extern "C" void phx_set_graphs(PhxCtx *c, int on) { c->fw.use_graphs = on != 0; }
 
-extern "C" int phx_set_track_lanes(PhxCtx *c, int lanes) {
-    PHX_REQUIRE(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 3 || lanes == 6,
-                "lanes per star must be 0 (chosen per pass), 1, 2, 3 or 6");
-    c->track_lanes = lanes;
+extern "C" int phx_set_track_offsets(PhxCtx *c, int offsets) {
+    PHX_REQUIRE(offsets >= 0 && offsets <= 3, "offsets per lane must be 1, 2 or 3 (0: the default, 2)");
+    c->track_offsets = offsets ? offsets : 2;
     return 0;
 }
 
@@ -407,14 +406,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     for (int i = 0; i < parts; i++) c->part_base[i] = parts > 1 ? (int)((long)(ns - last) * i / (parts - 1)) : 0;
     c->part_base[parts] = ns;
     if (parts == 1) {
-        // a pass run whole: six lanes per star while that gives at most one wavefront per SIMD, else three
-        // while that fits the two wavefronts per SIMD their registers allow, else two (the tracks are a chain
-        // of 2,000 sub-steps per lane; more lanes per star shorten it where SIMDs would idle).  Measured tracks
-        // of whole passes: 10,000 stars 1.66 ms with six lanes (1.99 with three, 2.45 with two); 20,000 stars
-        // 2.56 with three (3.21 with six, 3.07 with two); 40,000 stars 4.26 with three (5.27 with two)
-        const long lanes_max = 64L * 4 * c->fast.n_cu;
-        a.track_lanes = c->track_lanes ? c->track_lanes
-                        : (long)ns * 6 <= lanes_max ? 6 : (long)ns * 3 <= 2 * lanes_max ? 3 : 2;
+        a.track_offsets = c->track_offsets;
         return fast_screen(c->fast, c->fw, a, c->st, c->timing ? (StageMark)mark : nullptr, c);
     }
     if (!c->ev_fork) PHX_CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -441,9 +433,7 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
         ai.stars += b; ai.seeds += b; ai.prefix += b; ai.group += b; ai.nom += (size_t)b * 6;
         ai.ps0_in = c->ps0.as<double>() + (size_t)b * 6;
         ai.sig_in = c->sig.as<double>() + (size_t)b * 36;
-        // the last sub-pass's tracks with one lane per star: the sub-passes' tracks kernels then take fewer
-        // wavefronts than the device has SIMDs, so none shares a SIMD with another and none waits for one
-        ai.track_lanes = c->track_lanes ? c->track_lanes : i == parts - 1 ? 1 : 2;
+        ai.track_offsets = c->track_offsets;
         FastWork &W = part_fw(c, i);
         if (i > 0) {   // the launch settings of part 0
             W.scan_blocks = c->fw.scan_blocks; W.items_blocks = c->fw.items_blocks;
@@ -528,10 +518,11 @@ extern "C" int phx_kept(PhxCtx *c, long *n) {
 // (sub-passes overlapping), one knob after another with the others held, the mean of five passes per option
 // after one not counted, each pass checked complete (overlapping sub-passes vary from pass to pass: the best or
 // the median of a few would pick luck; throughput is the mean): the number of sub-passes and the last one's
-// share of the stars, the scan's first level (16-point coarse tiles or the 8-point tiles), its stars per task,
+// share of the stars, the tracks' offsets per lane, the scan's first level (16-point coarse tiles or the 8-point
+// tiles), its stars per task,
 // and the launch grids of the scan, the item evaluation, the kept-pair reduction and the finish.  Rows do not depend on any of them.  knob_ms[k]: the
 // best pass time while knob k was tuned.
-constexpr int N_KNOBS = 8;
+constexpr int N_KNOBS = 9;
 static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, double p_min, double knob_ms[N_KNOBS]) {
     PHX_REQUIRE(form == PHX_FORM_FAST || form == PHX_FORM_FAST64, "the autotuner tunes the fast forms");
     PHX_REQUIRE(!c->st_h.empty(), "load stars first");
@@ -541,6 +532,7 @@ static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, doubl
     struct Knob { int *v; int opts[5]; };
     Knob knobs[N_KNOBS] = {{&c->split, {3, 2, 4, 1, -1}},
                      {&c->last_pct, {40, 35, 45, 30, 50}},
+                     {&c->track_offsets, {2, 1, 3, -1, -1}},
                      {&c->fw.scan_coarse, {1, 0, -1, -1, -1}},
                      {&c->fw.scan_spw, {16, 8, -1, -1, -1}},
                      {&c->fw.scan_blocks, {512, 576, 768, 1024, 2048}},
@@ -608,6 +600,7 @@ extern "C" int phx_get_tuning(PhxCtx *c, PhxTuning *t) {
     t->coarse_tiles = c->fw.scan_coarse;
     t->sub_passes = c->split;
     t->last_share_pct = c->last_pct;
+    t->track_offsets = c->track_offsets;
     t->pass_ms = 0;
     return 0;
 }
@@ -629,12 +622,14 @@ extern "C" int phx_set_tuning(PhxCtx *c, const PhxTuning *t) {
     PHX_REQUIRE(t->sub_passes <= MAX_SPLIT, "sub_passes must be 1..4 (0 keeps the current setting)");
     PHX_REQUIRE(t->last_share_pct <= 0 || (t->last_share_pct >= 20 && t->last_share_pct <= 60),
                 "last_share_pct must be 20..60 (0 keeps the current setting)");
+    PHX_REQUIRE(t->track_offsets <= 3, "track_offsets must be 1..3 (0 keeps the current setting)");
     const int g[4] = {t->scan_blocks, t->items_blocks, t->reduce_blocks, t->finish_blocks};
     if (int rc = phx_set_grids(c, g)) return rc;
     c->fw.scan_spw = t->stars_per_task;
     c->fw.scan_coarse = t->coarse_tiles;
     if (t->sub_passes > 0) c->split = t->sub_passes;
     if (t->last_share_pct > 0) c->last_pct = t->last_share_pct;
+    if (t->track_offsets > 0) c->track_offsets = t->track_offsets;
     return 0;
 }
 

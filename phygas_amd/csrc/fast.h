@@ -50,8 +50,8 @@ struct FastArgs {
     // ps0 / Sigma0 of the batch's stars when already drawn (the sub-passes of a split pass share one draw
     // kernel); null: the pass draws them into its own buffers
     const double *ps0_in = nullptr, *sig_in = nullptr;
-    // lanes per star of the tracks kernel (2, or 1: all six offsets in one lane; the same bits either way)
-    int track_lanes = 2;
+    // offsets per lane of the tracks kernel's offset wavefronts (1, 2 or 3; the same bits whatever the value)
+    int track_offsets = 2;
 };
 
 typedef void (*StageMark)(void *ctx, const char *name);

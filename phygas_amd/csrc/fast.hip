@@ -3,11 +3,11 @@
 // Screening, per pass over a resident star batch (ns stars, real and control fused) against the resident
 // cluster tables:
 //   1. draws         ps0 / Sigma0 per star (the plain FP64 kernel; 64 draws per star)
-//   2. k_lin_fast    tracks: FP64 nominal orbit, the six neighbour offsets of the finite-difference STM
+//   2. k_lin_team    tracks: FP64 nominal orbit, the six neighbour offsets of the finite-difference STM
 //                    propagated with the FP32 difference force (force32.h) and FP64 accumulators,
-//                    S = A Sigma0 A^T assembled in FP64.  Two lanes per star, three offsets each (one lane
-//                    with all six in the last sub-pass of a split pass).  Writes the AoS tracks and their
-//                    tiled copy.
+//                    S = A Sigma0 A^T assembled in FP64.  One workgroup per 64 stars: one wavefront integrates
+//                    their nominal orbits (once per star) and the others their offsets, two per lane by
+//                    default.  Writes the AoS tracks and their tiled copy.
 //   3. k_star_recs   per (star, 8-point tile) and per (star, 16-point coarse tile): the tile records of
 //                    cull.h (axis line, tube radius and Loewner covariance bounds in the co-rotating frame:
 //                    one bound's largest eigenvalue for level 1, the interpolation between the tile's ends
@@ -117,17 +117,18 @@ __device__ __forceinline__ double rcp64(double b) {
 }
 
 // Acceleration with every root and quotient through rsq64 / rcp64 (u^-1.5 as (u^-1/2)^3) and the halo powers
-// from one log2 and two exp2; every operation in FP64 (a few ulp from the reference's pow-based evaluation).
-// inv_ha = 1 / halo_a.
+// from one log2 and one exp2; every operation in FP64 (a few ulp from the reference's pow-based evaluation).
+// inv_ha = 1 / halo_a.  Every fused multiply-add is written out and nothing else is contracted: a sum of two
+// products left to the compiler's contraction is fused on either product depending on the code around the
+// call, and kernels that integrate the same orbit must get the same bits.
 __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
-#pragma clang fp contract(fast)
-    const double R2 = p.x * p.x + p.y * p.y, r2 = R2 + p.z * p.z;
-    const double iu = rsq64(r2 + P.bulge_b * P.bulge_b);
+    const double R2 = __builtin_fma(p.y, p.y, p.x * p.x), r2 = __builtin_fma(p.z, p.z, R2);
+    const double iu = rsq64(__builtin_fma(P.bulge_b, P.bulge_b, r2));
     const double fb = -P.bulge_gm * (iu * iu * iu);
-    const double zz = p.z * p.z + P.disk_b * P.disk_b;
+    const double zz = __builtin_fma(p.z, p.z, P.disk_b * P.disk_b);
     const double izeta = rsq64(zz);
-    const double az = P.disk_a + zz * izeta;
-    const double iD = rsq64(R2 + az * az);
+    const double az = __builtin_fma(zz, izeta, P.disk_a);
+    const double iD = rsq64(__builtin_fma(az, az, R2));
     const double fd = -P.disk_gm * (iD * iD * iD);
     const double ir0 = rsq64(r2 > 0.0 ? r2 : 1.0), ir = r2 > 0.0 ? ir0 : 0.0;   // selects, no branch
     const double r = r2 * ir;
@@ -138,137 +139,191 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
     const double e1 = exp2((P.halo_gamma - 1.0) * lx);   // xh^(gamma - 1); xh^gamma = xh e1
     const double gm = P.halo_gm * (xh * e1) * rcp64(1.0 + e1);
     const double fh = -gm * (irs * irs * irs);
-    return v3((fb + fd + fh) * p.x, (fb + fd + fh) * p.y, (fb + fh) * p.z + fd * p.z * (az * izeta));
+    const double kxy = fb + fd + fh;
+    return v3(kxy * p.x, kxy * p.y, __builtin_fma(fb + fh, p.z, fd * p.z * (az * izeta)));
 }
 
 // ------------------------------------------------------------------------------------------- tracks
-// LPS lanes per star (1, 2, 3 or 6): lane q of a star propagates offsets NO q .. NO q + NO - 1 (NO = 6 / LPS;
-// offset o < 3 is position component o, o >= 3 velocity component o - 3) and every lane of the star runs the
-// same FP64 nominal.  Fewer lanes per star: fewer operations per star, a longer chain per lane; more lanes: the
-// other way round, for batches that would leave SIMDs idle.  All give the same bits: every offset's arithmetic
-// is the same.  A wavefront holds 64 / LPS whole stars (lanes past them idle), so the gather of the six
-// columns by lane 0 of a star stays inside the wavefront.  Track layout as the plain path: [star][npts][12] FP64.
-template <int LPS>
-__global__ __launch_bounds__(256) void k_lin_fast(const double *__restrict__ ps0, const double *__restrict__ sig,
-                                                  double *__restrict__ trk, Pot P, PotF Q, int npts, int n,
-                                                  double *__restrict__ tiled, int ntl) {
+// One workgroup per 64 stars (lane = star) in 1 + 6 / NO wavefronts with different roles:
+//   wavefront 0        the 64 FP64 nominal orbits, once per star; each sub-step's position, rounded to FP32 as
+//                      the difference force takes it, goes to an LDS ring; it also assembles S = A Sigma0 A^T
+//                      and writes the tracks
+//   wavefront w >= 1   offsets NO (w - 1) .. NO w - 1 of the same 64 stars (FP32 difference force, FP64
+//                      accumulators), reading the nominal's positions from the ring
+// The workgroup advances one point (PH_LIN_SUB sub-steps) per barrier with the nominal one point ahead: in step
+// j the nominal integrates point j + 1 and assembles S of point j - 1 from the columns of A that the offsets
+// left in LDS in step j - 1, while the offsets integrate point j.  Every wavefront passes the same npts + 1
+// barriers.  Every offset's arithmetic is the same whatever NO, so 1, 2 or 3 offsets per lane give the same
+// bits; running a star's nominal in a wavefront of its own takes the (1 + 6 / NO)-fold repetition of the FP64
+// orbit out of the offsets' lanes (a layout with the nominal on every lane of a star measured 4.5 against
+// 3.3 ms on 40,000 stars).  Kick-drift-kick sub-steps with each second half kick merged into the next first
+// one: K(h) D K(dt) D ... K(dt) D K(h); one check per sub-step sends the rare out-of-range terms of the
+// difference force to its branching form.
+template <int NO>
+__global__ __launch_bounds__(64 * (1 + 6 / NO)) __attribute__((amdgpu_waves_per_eu(3))) void k_lin_team(const double *__restrict__ ps0,
+                                                                const double *__restrict__ sig,
+                                                                double *__restrict__ trk, Pot P, PotF Q, int npts,
+                                                                int n, double *__restrict__ tiled, int ntl) {
 #pragma clang fp contract(fast)
-    static_assert(LPS == 1 || LPS == 2 || LPS == 3 || LPS == 6, "1, 2, 3 or 6 lanes per star");
-    constexpr int NO = 6 / LPS;     // offsets per lane
-    constexpr int SPW = 64 / LPS;   // stars per wavefront
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
-    const int s = (gid >> 6) * SPW + lane / LPS, q = lane % LPS;
-    const bool live = lane < SPW * LPS && s < n;
-    const int base = lane - q;      // lane 0 of the star
+    static_assert(NO == 1 || NO == 2 || NO == 3, "1, 2 or 3 offsets per lane");
+    __shared__ float ring[2][PH_LIN_SUB][3][64];   // nominal positions of the point being integrated
+    __shared__ double colA[2][18][64];             // A[r][o] = dx_o[r] / eps at entry 3 o + r
+    __shared__ double sg_s[36][64];                // Sigma0 of the workgroup's stars
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + lane;
+    const bool live = s < n;
+    // lanes past the batch's end integrate the last star's orbit without writing it (n > 0 on every launch)
+    const int se = live ? s : n - 1;
     const double eps = PH_LIN_EPS_POS, inv_eps = 1.0 / PH_LIN_EPS_POS, dt = -STEP_MYR / PH_LIN_SUB, h = 0.5 * dt;
-    // lanes without a star of their own (past the wavefront's whole stars or the batch's end) integrate a real
-    // star's orbit without writing it: a lane left at the origin would fail the step's range check every
-    // sub-step and hold the whole wavefront in the branching form of the difference force
-    V3 x = v3(0, 0, 0), v = v3(0, 0, 0);
-    if (n > 0) {
-        const double *p = ps0 + (size_t)min(s, n - 1) * 6;
-        x = v3(p[0], p[1], p[2]);
-        v = v3(p[3], p[4], p[5]);
-    }
-    double dx[NO][3], dv[NO][3];
+    const double *p0 = ps0 + (size_t)se * 6;
+    auto barrier = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    if (w == 0) {
+        const double *sgp = sig + (size_t)se * 36;
 #pragma unroll
-    for (int i = 0; i < NO; i++)
+        for (int i = 0; i < 36; i++) sg_s[i][lane] = sgp[i];
+        V3 x = v3(p0[0], p0[1], p0[2]), v = v3(p0[3], p0[4], p0[5]);
+        const double inv_ha = 1.0 / P.halo_a;
+        V3 a = accel_fast64(P, inv_ha, x);
+        // the tracks' position and velocity of point k (and their tiled copy: point k in tile k / 8, and the
+        // far end of tile k / 8 - 1; past the track the positions repeat the last one)
+        auto put_xv = [&](int k) {
+            if (!live) return;
+            double *out = trk + ((size_t)s * npts + k) * 12;
+            out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = v.x; out[4] = v.y; out[5] = v.z;
+            const int j = k >> 3, p = k & 7;
+            double *tb = tiled + ((size_t)s * ntl + j) * TREC;
+            tb[p] = x.x; tb[9 + p] = x.y; tb[18 + p] = x.z;
+            if (p == 0 && j > 0) { tb[8 - TREC] = x.x; tb[17 - TREC] = x.y; tb[26 - TREC] = x.z; }
+            if (k == npts - 1)
+                for (int pp = p + 1; pp <= TILE; pp++) { tb[pp] = x.x; tb[9 + pp] = x.y; tb[18 + pp] = x.z; }
+        };
+        put_xv(0);
+        for (int j = 0; j <= npts; j++) {
+            if (j >= 1 && live) {   // S of point j - 1 (past the track the tiled covariances are 0)
+                const int k = j - 1, sl = k & 1;
+                double A[3][6];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int o = NO * q + i;
-            dx[i][c] = (o < 3 && c == o) ? eps : 0.0;
-            dv[i][c] = (o >= 3 && c == o - 3) ? eps : 0.0;
-        }
-    const double inv_ha = 1.0 / P.halo_a;
-    V3 a = accel_fast64(P, inv_ha, x);
-    F3 da[NO];
+                for (int o = 0; o < 6; o++)
 #pragma unroll
-    for (int i = 0; i < NO; i++) da[i] = dforce32(Q, (float)x.x, (float)x.y, (float)x.z, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
-    for (int k = 0; k < npts; k++) {
-        if (k > 0) {
-            // kick-drift-kick sub-steps with each sub-step's second half kick merged into the next one's
-            // first (the same acceleration): K(h) D K(dt) D ... K(dt) D K(h)
-            for (int it = 0; it < PH_LIN_SUB; it++) {
-                const double kick = it == 0 ? h : dt;
-                v = v + a * kick;
+                    for (int r = 0; r < 3; r++) A[r][o] = colA[sl][3 * o + r][lane];
+                // A Sigma0 one column of Sigma0 at a time (the scheduler kept from loading all 36 entries ahead,
+                // which would hold 72 more registers); each entry summed over b = 0 .. 5 in order
+                double AS[3][6];
 #pragma unroll
-                for (int i = 0; i < NO; i++) {
-                    dv[i][0] += (double)da[i].x * kick; dv[i][1] += (double)da[i].y * kick;
-                    dv[i][2] += (double)da[i].z * kick;
+                for (int c = 0; c < 6; c++) {
+                    double g[6];
+#pragma unroll
+                    for (int b = 0; b < 6; b++) g[b] = sg_s[b * 6 + c][lane];
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int b = 0; b < 6; b++) acc += A[r][b] * g[b];
+                        AS[r][c] = acc;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                x = x + v * dt;
+                double S[6];
+                int o = 0;
 #pragma unroll
-                for (int i = 0; i < NO; i++)
+                for (int r = 0; r < 3; r++)
 #pragma unroll
-                    for (int c = 0; c < 3; c++) dx[i][c] += dv[i][c] * dt;
-                a = accel_fast64(P, inv_ha, x);
-                const float xf = (float)x.x, yf = (float)x.y, zf = (float)x.z;
-                // one branch per step for the rare terms outside the series' range or the halo cutoff
-                float tmax = 0.0f, r2max = 0.0f;
+                    for (int c = r; c < 3; c++) {
+                        double acc = 0.0;
 #pragma unroll
-                for (int i = 0; i < NO; i++)
-                    da[i] = dforce32_t<true>(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2], tmax,
-                                             r2max);
-                if (!(tmax < 1e-2f && r2max < Q.halo_cut2)) {
+                        for (int b = 0; b < 6; b++) acc += AS[r][b] * A[c][b];
+                        S[o++] = acc;
+                    }
+                double *out = trk + ((size_t)s * npts + k) * 12;
+#pragma unroll
+                for (int i = 0; i < 6; i++) out[6 + i] = S[i];
+                const int tj = k >> 3, p = k & 7;
+                double *tb = tiled + ((size_t)s * ntl + tj) * TREC;
+#pragma unroll
+                for (int f = 0; f < 6; f++) tb[27 + f * TILE + p] = S[f];
+                if (k == npts - 1)
+                    for (int pp = p + 1; pp < TILE; pp++)
+#pragma unroll
+                        for (int f = 0; f < 6; f++) tb[27 + f * TILE + pp] = 0.0;
+            }
+            if (j + 1 < npts) {   // point j + 1
+                const int sl = (j + 1) & 1;
+                for (int it = 0; it < PH_LIN_SUB; it++) {
+                    const double kick = it == 0 ? h : dt;
+                    v = v + a * kick;
+                    x = x + v * dt;
+                    a = accel_fast64(P, inv_ha, x);
+                    ring[sl][it][0][lane] = (float)x.x;
+                    ring[sl][it][1][lane] = (float)x.y;
+                    ring[sl][it][2][lane] = (float)x.z;
+                }
+                v = v + a * h;
+                put_xv(j + 1);
+            }
+            barrier();
+        }
+    } else {
+        const int o0 = NO * (w - 1);
+        double dx[NO][3], dv[NO][3];
+#pragma unroll
+        for (int i = 0; i < NO; i++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int o = o0 + i;
+                dx[i][c] = (o < 3 && c == o) ? eps : 0.0;
+                dv[i][c] = (o >= 3 && c == o - 3) ? eps : 0.0;
+            }
+        F3 da[NO];
+        {
+            const float xf = (float)p0[0], yf = (float)p0[1], zf = (float)p0[2];
+#pragma unroll
+            for (int i = 0; i < NO; i++)
+                da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+        }
+        for (int j = 0; j <= npts; j++) {
+            if (j >= 1 && j < npts) {
+                const int sl = j & 1;
+                for (int it = 0; it < PH_LIN_SUB; it++) {
+                    const double kick = it == 0 ? h : dt;
+#pragma unroll
+                    for (int i = 0; i < NO; i++) {
+                        dv[i][0] += (double)da[i].x * kick; dv[i][1] += (double)da[i].y * kick;
+                        dv[i][2] += (double)da[i].z * kick;
+                    }
 #pragma unroll
                     for (int i = 0; i < NO; i++)
-                        da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) dx[i][c] += dv[i][c] * dt;
+                    const float xf = ring[sl][it][0][lane], yf = ring[sl][it][1][lane], zf = ring[sl][it][2][lane];
+                    float tmax = 0.0f, r2max = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < NO; i++)
+                        da[i] = dforce32_t<true>(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2],
+                                                 tmax, r2max);
+                    if (!(tmax < 1e-2f && r2max < Q.halo_cut2)) {
+#pragma unroll
+                        for (int i = 0; i < NO; i++)
+                            da[i] = dforce32(Q, xf, yf, zf, (float)dx[i][0], (float)dx[i][1], (float)dx[i][2]);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < NO; i++) {
+                    dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h;
+                    dv[i][2] += (double)da[i].z * h;
                 }
             }
-            v = v + a * h;   // the last half kick: the velocities at the point
+            if (j < npts) {
+                const int sl = j & 1;
 #pragma unroll
-            for (int i = 0; i < NO; i++) {
-                dv[i][0] += (double)da[i].x * h; dv[i][1] += (double)da[i].y * h; dv[i][2] += (double)da[i].z * h;
+                for (int i = 0; i < NO; i++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) colA[sl][3 * (o0 + i) + r][lane] = dx[i][r] * inv_eps;
             }
-        }
-        // A[r][o] = dx[o][r] / eps: column o from lane o / NO of the star, its offset o % NO
-        double A[3][6];
-#pragma unroll
-        for (int o = 0; o < 6; o++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const double mine = dx[o % NO][r] * inv_eps;
-                A[r][o] = LPS == 1 ? mine : __shfl(mine, base + o / NO, 64);
-            }
-        if (!live || q != 0) continue;
-        const double *sg = sig + (size_t)s * 36;
-        double AS[3][6];
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = 0; c < 6; c++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int b = 0; b < 6; b++) acc += A[r][b] * sg[b * 6 + c];
-                AS[r][c] = acc;
-            }
-        double S[6];
-        int o = 0;
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = r; c < 3; c++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int b = 0; b < 6; b++) acc += AS[r][b] * A[c][b];
-                S[o++] = acc;
-            }
-        double *out = trk + ((size_t)s * npts + k) * 12;
-        out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = v.x; out[4] = v.y; out[5] = v.z;
-#pragma unroll
-        for (int i = 0; i < 6; i++) out[6 + i] = S[i];
-        // the tiled copy (k_tiled's layout): point k in tile k / 8, and the far end of tile k / 8 - 1
-        const int j = k >> 3, p = k & 7;
-        double *tb = tiled + ((size_t)s * ntl + j) * TREC;
-        tb[p] = x.x; tb[9 + p] = x.y; tb[18 + p] = x.z;
-#pragma unroll
-        for (int f = 0; f < 6; f++) tb[27 + f * TILE + p] = S[f];
-        if (p == 0 && j > 0) { tb[8 - TREC] = x.x; tb[17 - TREC] = x.y; tb[26 - TREC] = x.z; }
-        if (k == npts - 1) {   // past the track: positions repeat the last one, covariances are 0
-            for (int pp = p + 1; pp <= TILE; pp++) { tb[pp] = x.x; tb[9 + pp] = x.y; tb[18 + pp] = x.z; }
-            for (int pp = p + 1; pp < TILE; pp++)
-#pragma unroll
-                for (int f = 0; f < 6; f++) tb[27 + f * TILE + pp] = 0.0;
+            barrier();
         }
     }
 }
@@ -1404,12 +1459,12 @@ static int fast_launch_head(FastTables &T, FastWork &W, const FastArgs &a, hipSt
         hipLaunchKernelGGL(k_tiled, dim3(blocks_for((long)ns * ntl * TREC, 256)), dim3(256), 0, st, trk, nullptr, npts,
                            ns, npts, ntl, (double *)W.mem[W_TILED]);
     } else {   // writes the tiled copy too
-        const int lps = a.track_lanes;
-        PHX_REQUIRE(lps == 1 || lps == 2 || lps == 3 || lps == 6, "lanes per star must be 1, 2, 3 or 6");
-        auto kern = lps == 1 ? k_lin_fast<1> : lps == 2 ? k_lin_fast<2> : lps == 3 ? k_lin_fast<3> : k_lin_fast<6>;
-        const long waves = ((long)ns + 64 / lps - 1) / (64 / lps);   // whole stars per wavefront
-        hipLaunchKernelGGL(kern, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, st, ps0, sig, trk, a.P,
-                           make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
+        const int no = a.track_offsets;
+        PHX_REQUIRE(no == 1 || no == 2 || no == 3, "the tracks take 1, 2 or 3 offsets per lane");
+        auto kern = no == 1 ? k_lin_team<1> : no == 2 ? k_lin_team<2> : k_lin_team<3>;
+        if (ns > 0)
+            hipLaunchKernelGGL(kern, dim3(blocks_for(ns, 64)), dim3(64 * (1 + 6 / no)), 0, st, ps0, sig, trk, a.P,
+                               make_potf(a.P), npts, ns, (double *)W.mem[W_TILED], ntl);
     }
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "tracks");
@@ -1527,7 +1582,7 @@ int fast_screen(FastTables &T, FastWork &W, const FastArgs &a, hipStream_t st, S
     uint64_t key = 1469598103934665603ULL;
     auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ULL; };
     mix((uint64_t)a.ns); mix((uint64_t)a.nc); mix((uint64_t)a.npts); mix((uint64_t)a.n_samp);
-    mix((uint64_t)a.exact_tracks); mix((uint64_t)a.track_lanes); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
+    mix((uint64_t)a.exact_tracks); mix((uint64_t)a.track_offsets); mix((uint64_t)W.scan_blocks); mix((uint64_t)W.items_blocks);
     mix((uint64_t)W.reduce_blocks); mix((uint64_t)W.finish_blocks); mix((uint64_t)W.scan_spw);
     mix((uint64_t)W.scan_coarse);
     double dv[2] = {a.chi2_max, a.p_min};

@@ -85,7 +85,7 @@ def lib() -> C.CDLL:
         "phx_pair_lanes_for": (_I, [_P, _I, _L]),
         "phx_autotune_pairs": (_I, [_P, _I, _I, _P, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
         "phx_set_graphs": (None, [_P, _I]), "phx_graph_stats": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
-        "phx_set_split": (_I, [_P, _I]), "phx_set_track_lanes": (_I, [_P, _I]),
+        "phx_set_split": (_I, [_P, _I]), "phx_set_track_offsets": (_I, [_P, _I]),
         "phx_set_grids": (_I, [_P, _P]),
         "phx_autotune_ex": (_I, [_P, _I, _D, _I, _D, _P]),
         "phx_set_tuning": (_I, [_P, _P]), "phx_get_tuning": (_I, [_P, _P]),
@@ -159,7 +159,8 @@ class Tuning(C.Structure):
     """PhxTuning of include/phygas_hip.h."""
     _fields_ = [("scan_blocks", C.c_int), ("items_blocks", C.c_int), ("reduce_blocks", C.c_int),
                 ("finish_blocks", C.c_int), ("stars_per_task", C.c_int), ("coarse_tiles", C.c_int),
-                ("sub_passes", C.c_int), ("last_share_pct", C.c_int), ("pass_ms", C.c_double)]
+                ("sub_passes", C.c_int), ("last_share_pct", C.c_int), ("track_offsets", C.c_int),
+                ("pass_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -323,8 +324,9 @@ class Engine:
         return t.as_dict()
 
     def set_tuning(self, scan: int = 0, items: int = 0, reduce: int = 0, finish: int = 0, stars_per_task: int = 16,
-                   coarse_tiles: int = 1, sub_passes: int = 0, last_share_pct: int = 0):
-        t = Tuning(scan, items, reduce, finish, stars_per_task, coarse_tiles, sub_passes, last_share_pct, 0.0)
+                   coarse_tiles: int = 1, sub_passes: int = 0, last_share_pct: int = 0, track_offsets: int = 0):
+        t = Tuning(scan, items, reduce, finish, stars_per_task, coarse_tiles, sub_passes, last_share_pct,
+                   track_offsets, 0.0)
         _check(self._L.phx_set_tuning(self.h, C.byref(t)), "phx_set_tuning")
 
     def set_grids(self, scan: int = 0, items: int = 0, reduce: int = 0, finish: int = 0):
@@ -345,9 +347,9 @@ class Engine:
         """Sub-passes (1..4) of a fast pass over a large batch, each on its own stream."""
         _check(self._L.phx_set_split(self.h, int(parts)), "phx_set_split")
 
-    def set_track_lanes(self, lanes: int):
-        """Lanes per star of the fast tracks (1, 2, 3, 6; 0 = chosen per pass)."""
-        _check(self._L.phx_set_track_lanes(self.h, int(lanes)), "phx_set_track_lanes")
+    def set_track_offsets(self, offsets: int):
+        """Offsets per lane of the fast tracks' offset wavefronts (1, 2, 3; 0 = the default, 2)."""
+        _check(self._L.phx_set_track_offsets(self.h, int(offsets)), "phx_set_track_offsets")
 
     def graph_stats(self) -> tuple[int, int]:
         c, r = _L(0), _L(0)

@@ -381,10 +381,9 @@ def test_device_draws_match_host_reference(workload, oracle):
     assert np.array_equal(got["f_r50"], want["f_r50"])
 
 
-def test_track_lanes_give_the_same_bits(workload):
-    """The fast tracks with 1, 2, 3 or 6 lanes per star (the library's choice for a whole pass of this size: 6):
-    tracks, rows and work counts bit for bit the same.  171 stars + control leave the last wavefront of every lane
-    count partly empty."""
+def test_track_offsets_give_the_same_bits(workload):
+    """The fast tracks with 1, 2 or 3 offsets per lane (0 restores the default, 2): tracks, rows and work counts
+    bit for bit the same.  171 stars + control leave the last workgroup of 64 stars partly empty."""
     cl = workload.make_clusters(4087, 0)
     st = workload.make_stars(171, 2)
     ctl = workload.shuffled_control(st, 3)
@@ -394,12 +393,12 @@ def test_track_lanes_give_the_same_bits(workload):
         e.load_stars(ctl, 1)
         n = len(st) + len(ctl)
         ref = None
-        for lanes in (0, 1, 2, 3, 6):
-            e.set_track_lanes(lanes)
+        for offsets in (0, 1, 2, 3, 0):
+            e.set_track_offsets(offsets)
             rows = e.screen(hip.FAST, chi2_max=25.0, p_min=0.0)
             got = (rows.tobytes(), e.pass_stats(), e.star_tracks(0, n).tobytes())
             if ref is None:
                 ref = got
                 assert len(rows) > 100
-            assert got == ref, lanes
-        e.set_track_lanes(0)
+            assert got == ref, offsets
+        e.set_track_offsets(0)

@@ -92,10 +92,10 @@ int phx_kept(PhxCtx *ctx, long *n);
 /* Pairs the last pass evaluated point by point: every pair for the plain form, the pairs left by the tile
  * scan for the fast forms. */
 int phx_candidates(PhxCtx *ctx, long *n);
-/* Work counts of the last pass into st[0 .. min(cap, 6)): the scan's first-level tests (pair x 16-point
- * coarse tile), its certificate tests (coarse tiles and their 8-point tiles), candidate pairs,
- * point-evaluation items, kept pairs, rows.  For the plain form the first and third entries are the pair
- * count and the second and fourth 0. */
+/* Work counts of the last pass into st[0 .. min(cap, 7)): the scan's first-level tests (pair x 16-point
+ * coarse tile), its certificate tests (coarse tiles and their 8-point tiles), candidate pairs (any tile
+ * left), point-evaluation items (pair x tile left), kept pairs, rows, points evaluated.  For the plain form
+ * the first and third entries are the pair count and the second, fourth and seventh 0. */
 int phx_pass_stats(PhxCtx *ctx, long *st, int cap);
 /* Durations (ms, HIP events on the context's stream) of the last pass's stages; returns the count, names
  * into names[] (static strings). */

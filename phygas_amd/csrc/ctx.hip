@@ -636,7 +636,7 @@ extern "C" int phx_set_tuning(PhxCtx *c, const PhxTuning *t) {
 extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;
-    long v[6] = {0, 0, 0, 0, kept, nr};
+    long v[7] = {0, 0, 0, 0, kept, nr, 0};
     const long ns = (long)c->st_h.size();
     if (c->last_form == PHX_FORM_PLAIN) {
         v[0] = ns * c->nc;   // every pair is scanned point by point
@@ -647,9 +647,10 @@ extern "C" int phx_pass_stats(PhxCtx *c, long *st, int cap) {
             v[1] += part_fw(c, i).last_l2;
             v[2] += part_fw(c, i).last_cand;
             v[3] += part_fw(c, i).last_items;
+            v[6] += part_fw(c, i).last_points;
         }
     }
-    for (int i = 0; i < cap && i < 6; i++) st[i] = v[i];
+    for (int i = 0; i < cap && i < 7; i++) st[i] = v[i];
     return 0;
 }
 

@@ -63,9 +63,9 @@ struct Kept { int star, cluster, kb, pad; double sb, chi2; };
 enum { W_PS0, W_SIG, W_TRK, W_SREC1, W_CAND, W_KEPT, W_ROWS, W_CNT, W_ITEMS, W_IRES, W_ISTART,
        W_TILED, W_FSCR, W_FDRAW, W_FCNT, W_SURV, W_KSEG, W_SREC1C, W_SRECFC, W_SRECF, W_NSLOT };
 static_assert(W_NSLOT <= 24, "FastWork holds 24 buffers");
-// Counters (W_CNT): candidates, kept pairs, rows, items, (unused), level-2 tests of the scan, draws left
+// Counters (W_CNT): candidates, kept pairs, rows, items, points evaluated, level-2 tests of the scan, draws left
 // for the exact evaluation by the finish screen, largest kept segment, level-1 tests of the scan
-enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_UNUSED4, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
+enum { C_CAND, C_KEPT, C_ROWS, C_ITEMS, C_POINTS, C_L2, C_SURV, C_KSEGMAX, C_L1, C_N };
 // The reduce appends kept pairs to KSEG segments of the W_KSEG list, each with its own counter (counter i at
 // C_KSEG0 + i KSEG_STRIDE, one per 256 bytes): one counter for every wavefront of the grid would take its
 // atomics one after another.  k_kept_compact then packs the segments into W_KEPT.
@@ -704,7 +704,8 @@ constexpr int EXP_PER = 8;
 __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, const unsigned long long *__restrict__ ncand,
                                                 long cand_cap, Item *__restrict__ items,
                                                 unsigned long long *__restrict__ nitems, long item_cap,
-                                                long *__restrict__ istart) {
+                                                long *__restrict__ istart, const int *__restrict__ cn, int npts,
+                                                unsigned long long *__restrict__ npoints) {
     __shared__ unsigned wsum[4];
     __shared__ unsigned long long base_s;
     const long n = min((long)*ncand, cand_cap);
@@ -735,20 +736,26 @@ __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, c
         if (threadIdx.x == 0) base_s = atomicAdd(nitems, (unsigned long long)tot);
         __syncthreads();
         unsigned long long o = base_s + off + x - cnt;
+        unsigned pts = 0;   // points of the items (the evaluation's work count)
 #pragma unroll
         for (int u = 0; u < EXP_PER; u++) {
             const long id = b0 + threadIdx.x * EXP_PER + u;
             if (id < n) {
                 istart[id] = (long)o;
+                const int np = min(npts, cn[cd[u].cluster]);
                 unsigned long long mm = cd[u].mask;
                 while (mm) {
                     const int j = __builtin_ctzll(mm);
                     mm &= mm - 1ull;
                     if ((long)o < item_cap) items[o] = Item{(int)id, cd[u].star, cd[u].cluster, j};
                     o++;
+                    pts += (unsigned)max(0, min(TILE, np - j * TILE));
                 }
             }
         }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) pts += __shfl_xor(pts, d, 64);
+        if (lane == 0 && pts) atomicAdd(npoints, (unsigned long long)pts);
         __syncthreads();   // wsum / base_s reused by the next block of candidates
     }
 }
@@ -1510,7 +1517,7 @@ static int fast_launch_tail(FastTables &T, FastWork &W, const FastArgs &a, hipSt
     if (mark) mark(mark_ctx, "scan");
     const Cand *cand = (const Cand *)W.mem[W_CAND];
     hipLaunchKernelGGL(k_expand, dim3(2048), dim3(256), 0, st, cand, cnt + C_CAND, W.cand_cap, (Item *)W.mem[W_ITEMS],
-                       cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART]);
+                       cnt + C_ITEMS, W.item_cap, (long *)W.mem[W_ISTART], a.cn, npts, cnt + C_POINTS);
     PHX_CK(hipGetLastError());
     if (mark) mark(mark_ctx, "expand");
     hipLaunchKernelGGL(k_items, dim3(W.items_blocks > 0 ? W.items_blocks : 65536), dim3(256), 0, st,
@@ -1621,6 +1628,7 @@ int fast_counts(FastWork &W, hipStream_t st, long h[2], int *grow) {
     PHX_CK(hipStreamSynchronize(st));
     W.last_cand = (long)c[C_CAND];
     W.last_items = (long)c[C_ITEMS];
+    W.last_points = (long)c[C_POINTS];
     W.last_l2 = (long)c[C_L2];
     W.last_l1 = (long)c[C_L1];
     *grow = 0;

@@ -296,9 +296,9 @@ class Engine:
 
     def pass_stats(self) -> dict:
         """Work counts of the last pass (see phx_pass_stats)."""
-        st = np.zeros(6, np.int64)
-        _check(self._L.phx_pass_stats(self.h, st.ctypes.data, 6), "phx_pass_stats")
-        return dict(zip(("l1_tests", "l2_tests", "candidates", "items", "kept", "rows"), (int(x) for x in st)))
+        st = np.zeros(7, np.int64)
+        _check(self._L.phx_pass_stats(self.h, st.ctypes.data, 7), "phx_pass_stats")
+        return dict(zip(("l1_tests", "l2_tests", "candidates", "items", "kept", "rows", "points"), (int(x) for x in st)))
 
     def autotune(self, form: int = FAST, chi2_max: float = 25.0, n_samp: int = 64, p_min: float = 0.0) -> dict:
         """Pick the fast pass's launch grids by timing whole passes on the loaded batch; returns the choices and

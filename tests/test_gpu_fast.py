@@ -253,9 +253,10 @@ def test_autotune_keeps_rows_and_stats_consistent(unit):
                 assert e.screen(hip.FAST64, chi2_max=25.0, p_min=0.0).tobytes() == plain.tobytes(), (coarse, spw)
                 n = len(st) + len(ctl)
                 # level 1 on coarse tiles (two tiles each) or on every tile; level 2 on its survivors (and, with the
-                # coarse level, on the tiles of the coarse ones left)
+                # coarse level, on the tiles of the coarse ones left); at most 8 points per item
                 assert stats["l1_tests"] == n * (coarse_tiles if coarse else fine_tiles)
                 assert stats["candidates"] <= stats["items"] <= stats["l2_tests"] <= 3 * stats["l1_tests"]
+                assert stats["items"] < stats["points"] <= 8 * stats["items"]
                 assert stats["rows"] == len(before) and stats["kept"] >= stats["rows"]
                 if coarse in seen:   # stars per task change no test
                     assert stats == seen[coarse]

@@ -21,7 +21,7 @@ struct FastWork {
     size_t bytes[24] = {};
     long cand_cap = 0, kept_cap = 0, row_cap = 0, item_cap = 0, surv_cap = 0;
     long last_cand = 0, last_items = 0, last_l1 = 0, last_l2 = 0, last_points = 0;
-    // launch grids (0: the defaults, chosen on the 40,000-star x 4,087-cluster unit run as three sub-passes);
+    // launch grids (0: the defaults, chosen on the 40,000-star x 4,087-cluster unit run as sub-passes);
     // phx_autotune picks them by timing whole passes on the loaded batch
     int scan_blocks = 0, items_blocks = 0, reduce_blocks = 0, finish_blocks = 0;
     // the scan's stars per task (8 or 16) and its first level (1: 16-point coarse tiles, then their 8-point

@@ -199,6 +199,34 @@ def test_work_unit_inputs_against_the_oracle(unit, oracle):
     _compare(want, fast, 1e-4, "unit300_fast_vs_oracle")
 
 
+def test_work_unit_slice_against_the_oracle(workload, oracle):
+    """The first 2,000 stars of the 20,000-star work unit and their shuffled-velocity controls (the permutation of
+    the whole unit's triplets, seed 3) against the oracle: rows with p_r50 >= 0.001 of the plain FP64 form (the
+    1e-9 bar, kb and sb identical), of FAST64 (byte-identical to plain) and of the chosen form (the 1e-4 bar)."""
+    cl = workload.make_clusters(4087, 0)
+    st_all = workload.make_stars(20000, 2)
+    st, ctl = st_all[:2000], workload.shuffled_control(st_all, 3)[:2000]
+    nth = min(16, os.cpu_count() or 1)
+    w_real = oracle.screen(cl, st, chi2_max=25.0, nthreads=nth)
+    w_ctl = oracle.screen(cl, ctl, chi2_max=25.0, nthreads=nth)
+    w_ctl["star"] += len(st)
+    want = np.concatenate([w_real, w_ctl])
+    want = want[want["p_r50"] >= 0.001]
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.load_stars(ctl, 1)
+        plain = e.screen(hip.PLAIN, chi2_max=25.0, p_min=0.001)
+        fast64 = e.screen(hip.FAST64, chi2_max=25.0, p_min=0.001)
+        fast = e.screen(hip.FAST, chi2_max=25.0, p_min=0.001)
+    assert len(want) > 1000
+    a, b, common = _compare(want, plain, 1e-9, "unit2000_plain_vs_oracle")
+    for k in common:
+        assert a[k]["kb"] == b[k]["kb"] and a[k]["sb"] == b[k]["sb"], k
+    assert plain.tobytes() == fast64.tobytes()
+    _compare(want, fast, 1e-4, "unit2000_fast_vs_oracle")
+
+
 def test_full_work_unit_fast_and_fast64_against_plain(workload):
     """The whole work unit, 20,000 stars + control x 4,087 clusters: FAST64 rows byte-identical to the plain FP64
     form's, the chosen FAST form's within the bar of every common pair."""

@@ -7,6 +7,7 @@ with the cluster tables resident and the star records already in HBM.  The plain
 and the rows of the two compared.
 
 Further sections of the same JSON line:
+  roofline_items  the point evaluation (k_items) against the HBM peak: bytes from the profiled traffic record
   monte_carlo  1 pulsar x 300 prefiltered clusters x 200 clones, plain FP64 and FP32 clones, device draws
   two_stage    a 100,000-star shard (seed 4) with its control: the fast screening pass, then the Monte Carlo
                (1,000 clones, device draws, graph replay) of every real row with p_r50 >= 0.01
@@ -413,6 +414,7 @@ def main() -> int:
                  "max_abs_dp_r50": dp, "pairs_dp_r50_above_1e-4": nbad}
 
     # roofline of the dominant kernel (largest stage of the timed pass)
+    roof_items = None
     if form == hip.PLAIN:
         k_ms = stages.get("scan", float("nan"))
         flops = plain_scan_flops(cn, n_stars)
@@ -425,7 +427,7 @@ def main() -> int:
         name = max(("scan", "items"), key=lambda k: stages.get(k, 0.0))
         k_ms = stages.get(name, float("nan"))
         if name == "items":
-            work = stats["items"] * 8 * POINT_FLOP   # eight points per item (fewer at a track's end: an upper bound)
+            work = stats["points"] * POINT_FLOP   # the points the evaluation takes (counted on the device)
             achieved = work / (k_ms / 1000.0) / 1e12
             roof = {"bound": "valu", "kernel": "k_items", "achieved": achieved, "peak": PEAK_VALU,
                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_items"),
@@ -436,6 +438,17 @@ def main() -> int:
             roof = {"bound": "valu", "kernel": "k_scan_tiles", "achieved": achieved, "peak": PEAK_VALU,
                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU, "traffic": pmc_traffic("k_scan_tiles"),
                     "work_per_launch": work, "avg_launch_ms": k_ms}
+
+        # the point evaluation beside it: the reference's FP64 arithmetic on the points of the tiles the scan
+        # keeps, bound by the bytes it reads (two 600-byte tiles per item, mostly from beyond L2)
+        i_ms = stages.get("items", float("nan"))
+        i_work = stats["points"] * POINT_FLOP
+        i_bytes = pmc_traffic("k_items")
+        roof_items = {"bound": "hbm", "kernel": "k_items", "avg_launch_ms": i_ms, "points": stats["points"],
+                      "work_per_launch": i_work, "valu_tflops": i_work / (i_ms / 1000.0) / 1e12,
+                      "traffic": i_bytes, "unit": "GB/s", "peak": PEAK_HBM,
+                      "achieved": None if i_bytes is None else i_bytes / (i_ms / 1000.0) / 1e9}
+        roof_items["frac"] = None if i_bytes is None else roof_items["achieved"] / PEAK_HBM
 
     sub_passes = e.tuning()["sub_passes"] if args.split <= 0 else args.split
     mc = None if args.no_mc else monte_carlo(e, clusters, pulsar)
@@ -465,6 +478,8 @@ def main() -> int:
         "stages_ms": stages, "rows": result["rows"], "kept_pairs": kept, "pass_stats": stats, "autotune": tune,
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if roof_items:
+        line["roofline_items"] = roof_items
     if plain:
         line["plain"] = plain
         line["speedup_vs_plain"] = plain["ms"] / ms

@@ -160,10 +160,9 @@ __device__ __forceinline__ V3 accel_fast64(const Pot &P, double inv_ha, V3 p) {
 // one: K(h) D K(dt) D ... K(dt) D K(h); one check per sub-step sends the rare out-of-range terms of the
 // difference force to its branching form.
 template <int NO>
-__global__ __launch_bounds__(64 * (1 + 6 / NO)) __attribute__((amdgpu_waves_per_eu(3))) void k_lin_team(const double *__restrict__ ps0,
-                                                                const double *__restrict__ sig,
-                                                                double *__restrict__ trk, Pot P, PotF Q, int npts,
-                                                                int n, double *__restrict__ tiled, int ntl) {
+__global__ __launch_bounds__(64 * (1 + 6 / NO)) __attribute__((amdgpu_waves_per_eu(3)))
+void k_lin_team(const double *__restrict__ ps0, const double *__restrict__ sig, double *__restrict__ trk, Pot P,
+                PotF Q, int npts, int n, double *__restrict__ tiled, int ntl) {
 #pragma clang fp contract(fast)
     static_assert(NO == 1 || NO == 2 || NO == 3, "1, 2 or 3 offsets per lane");
     __shared__ float ring[2][PH_LIN_SUB][3][64];   // nominal positions of the point being integrated

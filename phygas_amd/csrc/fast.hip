@@ -1154,20 +1154,23 @@ __global__ __launch_bounds__(256) void k_fin_setup(const Kept *__restrict__ kept
 // Stage 1 hashes every draw's screening uniform (u of axis k) and queues the draws that pass the integer
 // bound; stage 2 takes 64 queued draws at a time, hashes their second uniform and sends those whose FP32 z_k
 // estimate lies in I_k (+- 1e-3) to the list of draws for k_fin_draw.  Both queues are per-wavefront LDS
-// rings of FQ entries with wave-uniform heads and tails; the list takes one atomic per 64 draws.
+// rings of FQ entries with wave-uniform heads and tails; the list takes one atomic per 64 draws.  About 13% of
+// the draws pass stage 1, spread so that 95% of a wavefront's stage-1 steps queue at least one: a queue entry
+// is one 16-byte record (the uniform's 53 bits, the draw's index) written at one address, and a lane's place
+// in the queue comes from mbcnt.
 constexpr int FQ = 128;
+struct FinQ { uint64_t w1; unsigned idx, pad; };
 __global__ __launch_bounds__(256) void k_finish_fast(const FinScr *__restrict__ fscr,
                                                      const unsigned long long *__restrict__ nkept, long kept_cap,
                                                      uint64_t *__restrict__ surv, unsigned long long *__restrict__ nsurv,
                                                      long surv_cap) {
     __shared__ FinScr par_s[4][8];
-    __shared__ uint64_t qw_s[4][FQ];
+    __shared__ FinQ q1_s[4][FQ];
     __shared__ uint64_t qx_s[4][FQ];
-    __shared__ unsigned short qi_s[4][FQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 3, p = lane & 7;
     FinScr *par = par_s[w];
-    uint64_t *qw = qw_s[w], *qx = qx_s[w];
-    unsigned short *qi = qi_s[w];
+    FinQ *q1 = q1_s[w];
+    uint64_t *qx = qx_s[w];
     const long nk = min((long)*nkept, kept_cap);
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwave = ((long)gridDim.x * blockDim.x) >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -1193,16 +1196,21 @@ __global__ __launch_bounds__(256) void k_finish_fast(const FinScr *__restrict__ 
         int h1 = 0, t1 = 0;
         // splitmix64 state before draw i's screening uniform: seed + (6 i + 2 k + 1) gamma, +48 gamma per step
         uint64_t st = q.seed + (uint64_t)(6 * p + 2 * (q.flags >> 2) + 1) * GOLDEN_GAMMA;
+        const unsigned gp = (unsigned)(g << 9 | p);   // a queue entry's index: g << 9 | draw (p + 8 t)
+        const unsigned long long live_mask = __builtin_amdgcn_ballot_w64(!none);
         wave_sync();
         for (int t = 0; t <= 64; t++, st += 48 * GOLDEN_GAMMA) {
             if (t < 64) {   // stage 1
                 const uint64_t w1 = mix64(st) >> 11;
                 const bool pass = !none && w1 <= u1max;   // a draw whose z_k lies outside I_k cannot count
-                const unsigned long long bal = __ballot(pass);
+                // the lanes of pass, as the comparison's own lane mask (ICmp predicate 37: unsigned <=)
+                const unsigned long long bal = __builtin_amdgcn_uicmpl(w1, u1max, 37) & live_mask;
                 if (pass) {
-                    const int pos = (t1 + __popcll(bal & below)) & (FQ - 1);
-                    qw[pos] = w1;
-                    qi[pos] = (unsigned short)(g << 9 | (p + 8 * t));
+                    const unsigned below_n = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                    FinQ &e = q1[(t1 + (int)below_n) & (FQ - 1)];
+                    e.w1 = w1;
+                    e.idx = gp + 8u * (unsigned)t;
                 }
                 t1 += __popcll(bal);
             }
@@ -1212,11 +1220,11 @@ __global__ __launch_bounds__(256) void k_finish_fast(const FinScr *__restrict__ 
                 bool pass = false;
                 uint64_t id = 0;
                 if (e < t1) {
-                    const int pos = e & (FQ - 1);
-                    const int v = qi[pos], gg = v >> 9, i = v & 511;
+                    const FinQ qe = q1[e & (FQ - 1)];
+                    const int v = (int)qe.idx, gg = v >> 9, i = v & 511;
                     const FinScr &f = par[gg];
                     const uint64_t b = f.seed + (uint64_t)(6 * i + 2 * (f.flags >> 2) + 2) * GOLDEN_GAMMA;
-                    const double u1 = (double)qw[pos] * (1.0 / 9007199254740992.0);
+                    const double u1 = (double)qe.w1 * (1.0 / 9007199254740992.0);
                     const float ze = z0_estimate(u1 <= 0.0 ? 1.0 / 18446744073709551616.0 : u1, to_unit(mix64(b)));
                     pass = (f.flags & 1) || (ze >= f.flo && ze <= f.fhi);
                     id = (uint64_t)(h0 + gg) << 9 | (uint64_t)i;

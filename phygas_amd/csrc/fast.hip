@@ -759,6 +759,27 @@ __global__ __launch_bounds__(256) void k_expand(const Cand *__restrict__ cand, c
     }
 }
 
+// The lexicographic (q, k) minimum over the 8 lanes of each group (lane >> 3), with its sv, through DPP moves
+// (VALU, no LDS round trip): lanes xor 1 and xor 2 within each quad, then each lane with its mirror in the
+// 8-lane half row, which lies in the other quad.  Every lane of the wavefront must be active.
+template <int CTRL> __device__ __forceinline__ unsigned dpp32(unsigned v) {
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL> __device__ __forceinline__ double dpp64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    return __builtin_bit_cast(double, (uint64_t)dpp32<CTRL>((unsigned)u) | (uint64_t)dpp32<CTRL>((unsigned)(u >> 32)) << 32);
+}
+template <int CTRL> __device__ __forceinline__ void qk_min_dpp(double &q, double &sv, int &k) {
+    const double qo = dpp64<CTRL>(q), so = dpp64<CTRL>(sv);
+    const int ko = (int)dpp32<CTRL>((unsigned)k);
+    if (qo < q || (qo == q && ko < k)) { q = qo; k = ko; sv = so; }
+}
+__device__ __forceinline__ void group8_min(double &q, double &sv, int &k) {
+    qk_min_dpp<0xB1>(q, sv, k);    // quad_perm [1, 0, 3, 2]
+    qk_min_dpp<0x4E>(q, sv, k);    // quad_perm [2, 3, 0, 1]
+    qk_min_dpp<0x141>(q, sv, k);   // row_half_mirror
+}
+
 // Eight lanes per item, one per point of the tile: the reference's FP64 evaluation, then the strict-<
 // first-index minimum over the tile (lexicographic (chi2, k) across the eight lanes).  The item list is cut
 // into 8 contiguous chunks, chunk x taken by the workgroups the dispatcher places on XCD x (workgroup b on
@@ -790,12 +811,7 @@ __global__ __launch_bounds__(256) void k_items(const Item *__restrict__ items, c
                 if (v >= 0) q = v;
             }
         }
-#pragma unroll
-        for (int d = 1; d < 8; d <<= 1) {
-            const double qo = __shfl_xor(q, d, 64), so = __shfl_xor(sv, d, 64);
-            const int ko = __shfl_xor(k, d, 64);
-            if (qo < q || (qo == q && ko < k)) { q = qo; k = ko; sv = so; }
-        }
+        group8_min(q, sv, k);
         if (p == 0 && id < c1) res[id] = IRes{q, sv, k, 0};
     }
 }
@@ -844,12 +860,7 @@ __global__ __launch_bounds__(256) void k_reduce(const Cand *__restrict__ cand, c
                 if (q.q < best || (q.q == best && q.k < kb)) { best = q.q; kb = q.k; sb = q.sv; }
             }
         }
-#pragma unroll
-        for (int d = 1; d < 8; d <<= 1) {
-            const double qo = __shfl_xor(best, d, 64), so = __shfl_xor(sb, d, 64);
-            const int ko = __shfl_xor(kb, d, 64);
-            if (qo < best || (qo == best && ko < kb)) { best = qo; kb = ko; sb = so; }
-        }
+        group8_min(best, sb, kb);
         const bool hit = id < n && p == 0 && best < chi2_max;
         const unsigned long long bal = __ballot(hit);
         if (hit) ring[(tail + __popcll(bal & below)) & (KQ - 1)] = Kept{cd.star, cd.cluster, kb, 0, sb, best};

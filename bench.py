@@ -7,7 +7,8 @@ with the cluster tables resident and the star records already in HBM.  The plain
 and the rows of the two compared.
 
 Further sections of the same JSON line:
-  roofline_items  the point evaluation (k_items) against the HBM peak: bytes from the profiled traffic record
+  roofline_items  the point evaluation (k_items): its fabric rate (bytes from the profiled traffic record) against
+               the HBM peak and its FP64 rate against the vector peak; latency-bound, neither roof binds
   monte_carlo  1 pulsar x 300 prefiltered clusters x 200 clones, plain FP64 and FP32 clones, device draws
   two_stage    a 100,000-star shard (seed 4) with its control: the fast screening pass, then the Monte Carlo
                (1,000 clones, device draws, graph replay) of every real row with p_r50 >= 0.01
@@ -440,12 +441,15 @@ def main() -> int:
                     "work_per_launch": work, "avg_launch_ms": k_ms}
 
         # the point evaluation beside it: the reference's FP64 arithmetic on the points of the tiles the scan
-        # keeps, bound by the bytes it reads (two 600-byte tiles per item, mostly from beyond L2)
+        # keeps.  It fetches ~720 bytes per item from beyond L2 at ~0.8 of the HBM peak, but those bytes do not
+        # bound it (items ordered by XCD and tile index cut them 15x and its time 4%): it waits on each lane's
+        # twenty tile loads and the chains of its unfused FP64 arithmetic
         i_ms = stages.get("items", float("nan"))
         i_work = stats["points"] * POINT_FLOP
         i_bytes = pmc_traffic("k_items")
-        roof_items = {"bound": "hbm", "kernel": "k_items", "avg_launch_ms": i_ms, "points": stats["points"],
+        roof_items = {"bound": "latency", "kernel": "k_items", "avg_launch_ms": i_ms, "points": stats["points"],
                       "work_per_launch": i_work, "valu_tflops": i_work / (i_ms / 1000.0) / 1e12,
+                      "valu_frac": i_work / (i_ms / 1000.0) / 1e12 / PEAK_VALU,
                       "traffic": i_bytes, "unit": "GB/s", "peak": PEAK_HBM,
                       "achieved": None if i_bytes is None else i_bytes / (i_ms / 1000.0) / 1e9}
         roof_items["frac"] = None if i_bytes is None else roof_items["achieved"] / PEAK_HBM

@@ -394,10 +394,11 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
 //            is wave-uniform and read through scalar loads);
 //   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
 //            evaluated 64 at a time, one per lane, with the cluster's coarse records staged in LDS;
-//   fine     the coarse tiles left, queued again, each entry testing tiles 2J and 2J + 1 with the fine
-//            level 2 (l2f_pass, covariances interpolated over the tile) on their own records (level 1
-//            would pass nearly all of them).  The verdicts return to the
-//            cluster's lane as (star, tile) bits.
+//   fine     the coarse tiles left, queued again as (star, cluster, J) in a second LDS ring that collects
+//            them across the task's coarse tiles and is evaluated 64 at a time (per coarse tile the fine
+//            passes would be 40% full), each entry testing tiles 2J and 2J + 1 with the fine level 2
+//            (l2f_pass, covariances interpolated over the tile) on their own records (level 1 would pass
+//            nearly all of them).  The verdicts go to the task's tile masks in LDS ([star][cluster]).
 // A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
 // drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
 // Coarse level-2 records of a wavefront's 64 clusters in LDS, field-major ([field][lane], the 24 FP32 fields
@@ -443,6 +444,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ float c2_s[4][CFF][64];
     __shared__ unsigned l2_s[4][64];
     __shared__ int nt_s[4][64];
+    constexpr int FQ2 = 128;   // fine ring: at most 63 entries wait between the coarse passes, each adding <= 64
+    __shared__ unsigned short fq_s[4][Coarse ? FQ2 : 1];
+    // the task's tile masks, [star][cluster lane], 32 tiles from tile mbase (64-bit masks: tiles 0-31, then
+    // 32-63 once the fine ring has been emptied and the first half moved to the lanes' masks)
+    __shared__ unsigned mask_s[4][Coarse ? SPWT : 1][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const float chi2cf = up(chi2c);   // the fine level's FP32 cut, rounded up
     unsigned short *q = q_s[w];
@@ -499,6 +505,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         MaskT mask[SPWT];
 #pragma unroll
         for (int i = 0; i < SPWT; i++) mask[i] = 0;
+        // the fine ring (entries J << 10 | star << 6 | cluster lane; head fh, tail ft, wave-uniform): one pass
+        // of up to 64 entries, the verdicts of tiles 2J and 2J + 1 or'ed into mask_s
+        unsigned short *fq = fq_s[w];
+        int fh = 0, ft = 0, mbase = 0;
+        auto fine_pass = [&]() {
+            wave_sync();
+            const int e = fh + lane;
+            if (e < ft) {
+                const int ent = fq[e & (FQ2 - 1)], J = ent >> 10, i = (ent >> 6) & 15, c = ent & 63;
+                const int ntc = nt_s[w][c], csc = cb * 64 + c;
+                unsigned bits = 0u;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int j = 2 * J + h;
+                    if (j < ntc) {
+                        fine_l2++;
+                        if (l2f_pass(srecf[(size_t)(s0 + i) * ntl + j], crecf[(size_t)j * nc_pad + csc], chi2cf))
+                            bits |= 1u << (j - mbase);
+                    }
+                }
+                if (bits) atomicOr(&mask_s[w][i][c], bits);
+            }
+            fh = min(fh + 64, ft);
+            wave_sync();
+        };
+        if constexpr (Coarse) {
+#pragma unroll
+            for (int i = 0; i < SPWT; i++) mask_s[w][i][lane] = 0;
+        }
         if constexpr (!Coarse) {
             // one level of 8-point tiles: level 1, then the certificate on the survivors, per tile j
             for (int j = 0; j < ntmax; j++) {
@@ -541,6 +576,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         for (int J = 0; Coarse && 2 * J < ntmax; J++) {
+            if (sizeof(MaskT) > 4 && 2 * J == 32) {   // wave-uniform: tiles 32-63 next
+                while (ft > fh) fine_pass();
+#pragma unroll
+                for (int i = 0; i < SPWT; i++) {
+                    mask[i] = mask_s[w][i][lane];
+                    mask_s[w][i][lane] = 0u;
+                }
+                mbase = 32;
+            }
             const bool cv = 2 * J < nt;
             const TileRec1 c1 = crec1c[(size_t)J * nc_pad + cs];
             const float t1 = cv ? tile_t1<TILE2>(ncl, J) : 0.0f;
@@ -563,48 +607,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             n_l2 += nq;
             enqueue(l1, o1);
             if (l1) lds_put_recf(c2s, lane, crecfc[(size_t)J * nc_pad + cs]);
-            l2_s[w][lane] = 0u;
             wave_sync();
+            const unsigned long long below = (1ull << lane) - 1ull;
             for (int e0 = 0; e0 < nq; e0 += 64) {   // wave-uniform
                 const int e = e0 + lane;
+                bool pass = false;
+                int ent = 0;
                 if (e < nq) {
-                    const int ent = q[e], i = ent >> 6, c = ent & 63;
-                    if (l2f_pass<TILE2>(srecfc[(size_t)(s0 + i) * ntlc + J], lds_get_recf(c2s, c), chi2cf))
-                        atomicOr(&l2_s[w][c], 1u << i);
+                    ent = q[e];
+                    const int i = ent >> 6, c = ent & 63;
+                    pass = l2f_pass<TILE2>(srecfc[(size_t)(s0 + i) * ntlc + J], lds_get_recf(c2s, c), chi2cf);
                 }
+                // the coarse tiles left go to the fine ring
+                const unsigned long long bal = __ballot(pass);
+                if (pass) fq[(ft + __popcll(bal & below)) & (FQ2 - 1)] = (unsigned short)(J << 10 | ent);
+                ft += __popcll(bal);
+                if (ft - fh >= 64) fine_pass();   // wave-uniform
             }
-            wave_sync();
-            const unsigned lc = l2_s[w][lane];
-            wave_sync();   // q / l2_s are rewritten below
-            // fine level 2 on tiles 2J, 2J + 1 of the coarse tiles left
-            int nq2 = 0;
-            const unsigned o2 = excl_prefix((unsigned)__popc(lc), &nq2);
-            if (nq2 == 0) continue;
-            enqueue(lc, o2);
-            l2_s[w][lane] = 0u;
-            wave_sync();
-            for (int e0 = 0; e0 < nq2; e0 += 64) {   // wave-uniform
-                const int e = e0 + lane;
-                if (e < nq2) {
-                    const int ent = q[e], i = ent >> 6, c = ent & 63;
-                    const int ntc = nt_s[w][c], csc = cb * 64 + c;
+            wave_sync();   // q / c2s are rewritten by the next coarse tile
+        }
+        if constexpr (Coarse) {
+            while (ft > fh) fine_pass();   // wave-uniform
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const int j = 2 * J + h;
-                        if (j < ntc) {
-                            fine_l2++;
-                            if (l2f_pass(srecf[(size_t)(s0 + i) * ntl + j], crecf[(size_t)j * nc_pad + csc], chi2cf))
-                                atomicOr(&l2_s[w][c], 1u << (2 * i + h));
-                        }
-                    }
-                }
-            }
-            wave_sync();
-            const unsigned l2 = l2_s[w][lane];
-#pragma unroll
-            for (int i = 0; i < SPWT; i++)
-                mask[i] |= (MaskT)((l2 >> (2 * i)) & 3u) << (2 * J);
-            wave_sync();   // q / c2s / l2_s are rewritten by the next coarse tile
+            for (int i = 0; i < SPWT; i++) mask[i] |= (MaskT)mask_s[w][i][lane] << (mbase & (8 * sizeof(MaskT) - 1));
         }
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) fine_l2 += __shfl_xor(fine_l2, d, 64);

@@ -392,8 +392,8 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
 // nothing but the order of the candidate list.  Per coarse tile J (TILE2 = 2 TILE points: tiles 2J, 2J + 1):
 //   level 1  every star of the block against the lane's cluster, on the coarse records (the star's record
 //            is wave-uniform and read through scalar loads);
-//   level 2  the level-1 survivors of the whole wavefront, queued in LDS as (star, cluster) entries and
-//            evaluated 64 at a time, one per lane, with the cluster's coarse records staged in LDS;
+//   level 2  the level-1 survivors of the whole wavefront, queued in an LDS ring as (star, cluster, J)
+//            entries across the task's coarse tiles and evaluated 64 at a time, one per lane;
 //   fine     the coarse tiles left, queued again as (star, cluster, J) in a second LDS ring that collects
 //            them across the task's coarse tiles and is evaluated 64 at a time (per coarse tile the fine
 //            passes would be 40% full), each entry testing tiles 2J and 2J + 1 with the fine level 2
@@ -401,9 +401,10 @@ __global__ __launch_bounds__(256) void k_cluster_recs(const double *__restrict__
 //            nearly all of them).  The verdicts go to the task's tile masks in LDS ([star][cluster]).
 // A coarse tile dropped holds no point under the cut, so neither of its tiles can (the fine tests only
 // drop more).  Pairs with any tile left are appended with the bit mask of those tiles (MaskT: 32 or 64).
-// Coarse level-2 records of a wavefront's 64 clusters in LDS, field-major ([field][lane], the 24 FP32 fields
-// of TileRecF): the lanes' writes are consecutive and reads of different clusters fall in different banks,
-// where a [lane] array of 96-byte records would put the lanes' copies of a field in few banks.
+// Without the coarse level, the level-2 records of a wavefront's 64 clusters are staged in LDS field-major
+// ([field][lane], the 24 FP32 fields of TileRecF): the lanes' writes are consecutive and reads of different
+// clusters fall in different banks, where a [lane] array of 96-byte records would put the lanes' copies of a
+// field in few banks.
 constexpr int CFF = sizeof(TileRecF) / sizeof(float);
 static_assert(CFF == 24, "TileRecF is 24 floats");
 __device__ __forceinline__ void lds_put_recf(float (*d)[64], int i, const TileRecF &r) {
@@ -440,8 +441,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     unsigned long long *__restrict__ n_l1_total,
                                                     unsigned long long *__restrict__ n_l2_total) {
     static_assert(SPWT == 8 || SPWT == 16, "8 or 16 stars per task");
-    __shared__ unsigned short q_s[4][SPWT * 64];
-    __shared__ float c2_s[4][CFF][64];
+    // the level-1 survivors: per tile without the coarse level; with it, a ring across the task's coarse
+    // tiles (at most 63 entries wait while a coarse tile adds at most SPWT x 64)
+    constexpr int QN = Coarse ? 2048 : SPWT * 64;
+    static_assert(!Coarse || QN >= 63 + SPWT * 64, "coarse ring");
+    __shared__ unsigned short q_s[4][QN];
+    __shared__ float c2_s[4][CFF][Coarse ? 1 : 64];
     __shared__ unsigned l2_s[4][64];
     __shared__ int nt_s[4][64];
     constexpr int FQ2 = 128;   // fine ring: at most 63 entries wait between the coarse passes, each adding <= 64
@@ -452,7 +457,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const float chi2cf = up(chi2c);   // the fine level's FP32 cut, rounded up
     unsigned short *q = q_s[w];
-    float (*c2s)[64] = c2_s[w];
     const int ncb = (nc + 63) / 64, nsb = (ns + SPWT - 1) / SPWT;
     const int xcd = blockIdx.x % N_XCD;
     int qi = 0;   // queues tried: own, then the following ones (wave-uniform)
@@ -467,11 +471,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         *tot = __builtin_amdgcn_readfirstlane(__shfl(x, 63, 64));
         return x - cnt;
     };
-    auto enqueue = [&](unsigned bits, unsigned o) {
+    auto enqueue = [&](unsigned bits, unsigned o, int tag) {   // entries tag << 10 | star << 6 | lane
         while (bits) {
             const int i = __builtin_ctz(bits);
             bits &= bits - 1u;
-            q[o++] = (unsigned short)(i << 6 | lane);
+            q[o++ & (QN - 1)] = (unsigned short)(tag << 10 | i << 6 | lane);
         }
     };
     for (;;) {
@@ -509,6 +513,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // of up to 64 entries, the verdicts of tiles 2J and 2J + 1 or'ed into mask_s
         unsigned short *fq = fq_s[w];
         int fh = 0, ft = 0, mbase = 0;
+        int qh = 0, qt = 0;   // the coarse ring's head and tail (wave-uniform)
         auto fine_pass = [&]() {
             wave_sync();
             const int e = fh + lane;
@@ -530,11 +535,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             fh = min(fh + 64, ft);
             wave_sync();
         };
+        // one coarse level-2 pass over up to 64 entries of the coarse ring; the coarse tiles left go to the
+        // fine ring, which takes a pass of its own whenever 64 wait
+        const unsigned long long below = (1ull << lane) - 1ull;
+        auto coarse_pass = [&]() {
+            wave_sync();
+            const int e = qh + lane;
+            bool pass = false;
+            int ent = 0;
+            if (e < qt) {
+                ent = q[e & (QN - 1)];
+                const int J = ent >> 10, i = (ent >> 6) & 15, c = ent & 63;
+                pass = l2f_pass<TILE2>(srecfc[(size_t)(s0 + i) * ntlc + J], crecfc[(size_t)J * nc_pad + cb * 64 + c],
+                                       chi2cf);
+            }
+            const unsigned long long bal = __ballot(pass);
+            if (pass) fq[(ft + __popcll(bal & below)) & (FQ2 - 1)] = (unsigned short)ent;
+            ft += __popcll(bal);
+            qh = min(qh + 64, qt);
+            if (ft - fh >= 64) fine_pass();   // wave-uniform
+        };
         if constexpr (Coarse) {
 #pragma unroll
             for (int i = 0; i < SPWT; i++) mask_s[w][i][lane] = 0;
         }
         if constexpr (!Coarse) {
+            float (*c2s)[64] = c2_s[w];
             // one level of 8-point tiles: level 1, then the certificate on the survivors, per tile j
             for (int j = 0; j < ntmax; j++) {
                 const bool cv = j < nt;
@@ -556,7 +582,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const unsigned o1 = excl_prefix((unsigned)__popc(l1), &nq);
                 if (nq == 0) continue;
                 n_l2 += nq;
-                enqueue(l1, o1);
+                enqueue(l1, o1, 0);
                 if (l1) lds_put_recf(c2s, lane, crecf[(size_t)j * nc_pad + cs]);
                 l2_s[w][lane] = 0u;
                 wave_sync();
@@ -577,6 +603,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         for (int J = 0; Coarse && 2 * J < ntmax; J++) {
             if (sizeof(MaskT) > 4 && 2 * J == 32) {   // wave-uniform: tiles 32-63 next
+                while (qt > qh) coarse_pass();
                 while (ft > fh) fine_pass();
 #pragma unroll
                 for (int i = 0; i < SPWT; i++) {
@@ -600,34 +627,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if (l1_pass<TILE2>(sr[u], c1, t1, kchi)) l1 |= 1u << (i0 + u);
             }
             if (!cv) l1 = 0u;
-            // coarse level 2 over the wavefront's level-1 survivors
+            // the wavefront's level-1 survivors to the coarse ring
             int nq = 0;
             const unsigned o1 = excl_prefix((unsigned)__popc(l1), &nq);
             if (nq == 0) continue;
             n_l2 += nq;
-            enqueue(l1, o1);
-            if (l1) lds_put_recf(c2s, lane, crecfc[(size_t)J * nc_pad + cs]);
-            wave_sync();
-            const unsigned long long below = (1ull << lane) - 1ull;
-            for (int e0 = 0; e0 < nq; e0 += 64) {   // wave-uniform
-                const int e = e0 + lane;
-                bool pass = false;
-                int ent = 0;
-                if (e < nq) {
-                    ent = q[e];
-                    const int i = ent >> 6, c = ent & 63;
-                    pass = l2f_pass<TILE2>(srecfc[(size_t)(s0 + i) * ntlc + J], lds_get_recf(c2s, c), chi2cf);
-                }
-                // the coarse tiles left go to the fine ring
-                const unsigned long long bal = __ballot(pass);
-                if (pass) fq[(ft + __popcll(bal & below)) & (FQ2 - 1)] = (unsigned short)(J << 10 | ent);
-                ft += __popcll(bal);
-                if (ft - fh >= 64) fine_pass();   // wave-uniform
-            }
-            wave_sync();   // q / c2s are rewritten by the next coarse tile
+            enqueue(l1, (unsigned)qt + o1, J);
+            qt += nq;
+            while (qt - qh >= 64) coarse_pass();   // wave-uniform
         }
         if constexpr (Coarse) {
-            while (ft > fh) fine_pass();   // wave-uniform
+            while (qt > qh) coarse_pass();   // wave-uniform
+            while (ft > fh) fine_pass();
 #pragma unroll
             for (int i = 0; i < SPWT; i++) mask[i] |= (MaskT)mask_s[w][i][lane] << (mbase & (8 * sizeof(MaskT) - 1));
         }
@@ -646,7 +657,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(ncand, (unsigned long long)total);
         base = __shfl(base, 0, 64);
-        const unsigned long long below = (1ull << lane) - 1ull;
         const int cl = cs < nc ? perm[cs] : 0;
 #pragma unroll
         for (int i = 0; i < SPWT; i++) {

@@ -105,9 +105,10 @@ void phx_set_timing(PhxCtx *ctx, int on);
  * same (a pass with timing on runs eagerly, its stage events between the kernels). */
 void phx_set_graphs(PhxCtx *ctx, int on);
 int phx_graph_stats(PhxCtx *ctx, long *captures, long *replays);
-/* A fast pass over a batch of at least 4,096 stars runs as `parts` sub-passes (1..4; default 2) over
+/* A fast pass over a batch of at least 4,096 stars runs as `parts` sub-passes (1..4; default 3) over
  * contiguous star ranges, each on its own stream, so that one's latency-bound tracks overlap another's scan;
- * rows are the same.  A pass with timing on runs whole.
+ * the last takes last_share_pct of the stars (phx_set_tuning, default 35%) and the others share the rest
+ * equally.  Rows are the same.  A pass with timing on runs whole.
  * No counterpart in the reference (its screen is one host loop over the targets, src/phygas_screen.c:48-67). */
 int phx_set_split(PhxCtx *ctx, int parts);
 /* Offsets per lane of the fast tracks kernel (1, 2 or 3; 0 restores the default, 2).  The kernel runs one
@@ -129,8 +130,8 @@ int phx_set_grids(PhxCtx *ctx, const int *grids);
 /* Every tunable setting of the fast pass.  stars_per_task: stars of one scan task (8 or 16; each task is one
  * wavefront over 64 clusters of similar track length, the clusters sorted by length into buckets of 64);
  * coarse_tiles: the scan's first level on 16-point time tiles, then their 8-point tiles (1, default), or on
- * the 8-point tiles directly (0); sub_passes: as phx_set_split (1..4, default 2); last_share_pct: the last
- * sub-pass's share of the stars (20..60 %, default 50); track_offsets: as phx_set_track_offsets (1..3, default
+ * the 8-point tiles directly (0); sub_passes: as phx_set_split (1..4, default 3); last_share_pct: the last
+ * sub-pass's share of the stars (20..60 %, default 35); track_offsets: as phx_set_track_offsets (1..3, default
  * 2).  In phx_set_tuning sub_passes, last_share_pct and track_offsets <= 0 keep the current setting.  Rows do
  * not depend on any setting.  No counterpart in the reference. */
 typedef struct {

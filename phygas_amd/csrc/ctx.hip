@@ -75,8 +75,8 @@ struct PhxCtx {
     double last_chi2_max = 25.0, last_p_min = 0.0;
     int last_n_samp = 64;
     FastWork fw;       // fast-form work buffers (part 0 of a split pass)
-    int split = 2;     // sub-passes of a large fast pass (phx_set_split)
-    int last_pct = 50; // the last sub-pass's share of the stars, percent (phx_set_tuning)
+    int split = 3;     // sub-passes of a large fast pass (phx_set_split)
+    int last_pct = 35; // the last sub-pass's share of the stars, percent (phx_set_tuning)
     int track_offsets = 2; // offsets per lane of the fast tracks' offset wavefronts (phx_set_track_offsets)
     int nparts = 1;    // sub-passes of the last fast pass, over stars [part_base[i], part_base[i + 1])
     int part_base[MAX_SPLIT + 1] = {};
@@ -398,9 +398,9 @@ static int screen_pass(PhxCtx *c, int form, double chi2_max, int n_samp, double 
     const int ns = a.ns;
     int parts = c->timing || ns < 2 * SPLIT_MIN ? 1 : std::min(c->split, MAX_SPLIT);
     c->nparts = parts;
-    // the last sub-pass takes last_pct of the stars and the others share the rest equally; by default two
-    // halves: measured on the 40,000-star unit, 15.4-15.8 ms per pass against 15.9-16.2 for three sub-passes
-    // of 30/30/40% (the pass time moves with how the sub-passes' kernels meet, not smoothly with the sizes;
+    // the last sub-pass takes last_pct of the stars and the others share the rest equally; by default three
+    // sub-passes of 32.5/32.5/35%: measured on the 40,000-star unit, 14.5-14.7 ms per pass against 15.0-15.2
+    // for two halves (the pass time moves with how the sub-passes' kernels meet, not smoothly with the sizes;
     // the autotuner tries other counts and shares on the caller's batch)
     const int last = parts > 1 ? (int)((long)ns * c->last_pct / 100) : ns;
     for (int i = 0; i < parts; i++) c->part_base[i] = parts > 1 ? (int)((long)(ns - last) * i / (parts - 1)) : 0;
@@ -530,8 +530,8 @@ static int autotune_fast(PhxCtx *c, int form, double chi2_max, int n_samp, doubl
     long kept, nr;
     if (last_counts(c, &kept, &nr)) return 1;   // sizes the buffers
     struct Knob { int *v; int opts[5]; };
-    Knob knobs[N_KNOBS] = {{&c->split, {2, 3, 4, 1, -1}},
-                     {&c->last_pct, {50, 45, 40, 55, 35}},
+    Knob knobs[N_KNOBS] = {{&c->split, {3, 2, 4, 1, -1}},
+                     {&c->last_pct, {35, 40, 45, 50, 30}},
                      {&c->track_offsets, {2, 1, 3, -1, -1}},
                      {&c->fw.scan_coarse, {1, 0, -1, -1, -1}},
                      {&c->fw.scan_spw, {16, 8, -1, -1, -1}},

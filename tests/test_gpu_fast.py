@@ -340,7 +340,7 @@ def test_split_passes_give_the_same_rows(workload):
             e.set_tuning(sub_passes=parts, last_share_pct=share)
             rows = [e.screen(hip.FAST, chi2_max=25.0, p_min=p) for p in (0.0, 0.001)]
             out[(parts, share)] = (rows, e.pass_stats(), e.star_tracks(1240, 20))
-        e.set_tuning(sub_passes=2, last_share_pct=50)
+        e.set_tuning(sub_passes=3, last_share_pct=35)
     ref_rows, ref_stats, ref_trk = out[(1, False)]
     assert len(ref_rows[0]) > 1000 and len(ref_rows[1]) > 50
     assert ref_trk.shape == (20, npts, 12)

@@ -317,6 +317,35 @@ def test_fast64_edge_batches_identical_to_plain(workload, ns):
                 assert len(plain) == 0
 
 
+def test_long_tracks_wide_tile_masks(workload, oracle):
+    """Trace-backs of 120-250 Myr (up to 501 points, 63 tiles): the scan's 64-bit tile masks, whose LDS half
+    moves to tiles 32-63 once the first half is done.  FAST64 rows equal PLAIN's byte for byte, and PLAIN's
+    the oracle's on the first 40 stars."""
+    cl = workload.make_clusters(400, 0)
+    rng = np.random.default_rng(7)
+    cl["tmax_myr"] = rng.uniform(120.0, 250.0, len(cl))   # 241-501 points
+    cl["tmax_myr"][:3] = [250.0, 200.0, 100.0]
+    st = workload.make_stars(300, 2)
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        assert e.npts == 501
+        for chi2_max, p_min in ((25.0, 0.0), (100.0, 0.001)):
+            plain = e.screen(hip.PLAIN, chi2_max=chi2_max, p_min=p_min)
+            kp = e.kept()
+            f64 = e.screen(hip.FAST64, chi2_max=chi2_max, p_min=p_min)
+            kf = e.kept()
+            assert kp == kf and plain.tobytes() == f64.tobytes(), (chi2_max, p_min)
+            if chi2_max == 25.0:
+                assert len(plain) > 100 and int(plain["kb"].max()) >= 8 * 32   # minima past tile 32
+                sub = plain[plain["star"] < 40]
+                want = oracle.screen(cl, st[:40], chi2_max=25.0, use_cut=True)
+                assert len(sub) == len(want) > 10
+                assert np.array_equal(sub["star"], want["star"]) and np.array_equal(sub["cluster"], want["cluster"])
+                assert np.array_equal(sub["kb"], want["kb"]) and np.array_equal(sub["p_r50"], want["p_r50"])
+                assert np.allclose(sub["chi2"], want["chi2"], rtol=1e-9, atol=1e-12)
+
+
 def test_split_passes_give_the_same_rows(workload):
     """A pass split into 2, 3 or 4 sub-passes on their own streams, with the last one's share of the stars from 30
     to 60%, gives the whole pass's rows, work counts and tracks (also with hipGraph replays)."""

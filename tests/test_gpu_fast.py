@@ -270,7 +270,7 @@ def test_autotune_keeps_rows_and_stats_consistent(unit):
         _report("autotune_stars_per_task", tune["stars_per_task"])
         _report("autotune_coarse_tiles", tune["coarse_tiles"])
         assert tune["stars_per_task"] in (8, 16) and tune["coarse_tiles"] in (0, 1) and tune["pass_ms"] > 0
-        assert 1 <= tune["sub_passes"] <= 4 and 35 <= tune["last_share_pct"] <= 55
+        assert 1 <= tune["sub_passes"] <= 4 and 30 <= tune["last_share_pct"] <= 50   # the tuner's options
         seen = {}
         for coarse in (1, 0):
             for spw in (16, 8):

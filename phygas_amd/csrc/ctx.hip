@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -675,23 +676,37 @@ extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
         PHX_CK(hipMemcpyAsync(rows, c->rows.p, sizeof(PhxRow) * m, hipMemcpyDeviceToHost, c->st));
         PHX_CK(hipStreamSynchronize(c->st));
     } else if (m > 0) {
-        // every part's rows (its star indices are local to the part, its rows in append order), sorted into
-        // (star, cluster) order before the first m are returned
-        std::vector<PhxRow> all(nr);
+        // every part's rows (its star indices are local to the part, its rows in append order), put into
+        // (star, cluster) order before the first m are returned: counted by star, then each star's few rows
+        // ordered by cluster ((star, cluster) is unique per row)
+        std::unique_ptr<PhxRow[]> all(new PhxRow[nr]);
         long o = 0;
         for (int i = 0; i < c->nparts; i++) {
             const long mi = c->part_rows[i];
             if (mi <= 0) continue;
-            PHX_CK(hipMemcpyAsync(all.data() + o, fast_rows_ptr(part_fw(c, i)), sizeof(PhxRow) * mi,
+            PHX_CK(hipMemcpyAsync(all.get() + o, fast_rows_ptr(part_fw(c, i)), sizeof(PhxRow) * mi,
                                   hipMemcpyDeviceToHost, c->st));
             PHX_CK(hipStreamSynchronize(c->st));
             for (long r = o; r < o + mi; r++) all[r].star += c->part_base[i];
             o += mi;
         }
-        std::sort(all.begin(), all.begin() + o, [](const PhxRow &a, const PhxRow &b) {
-            return a.star != b.star ? a.star < b.star : a.cluster < b.cluster;
-        });
-        memcpy(rows, all.data(), sizeof(PhxRow) * (size_t)m);
+        const int ns = (int)c->st_h.size();
+        std::vector<long> at(ns + 1, 0);
+        for (long r = 0; r < o; r++) {
+            PHX_REQUIRE(all[r].star >= 0 && all[r].star < ns, "row star index out of the batch");
+            at[all[r].star + 1]++;
+        }
+        for (int s = 0; s < ns; s++) at[s + 1] += at[s];
+        std::vector<long> idx(o);
+        {
+            std::vector<long> put(at.begin(), at.end() - 1);
+            for (long r = 0; r < o; r++) idx[put[all[r].star]++] = r;
+        }
+        for (int s = 0; s < ns; s++)
+            if (at[s + 1] - at[s] > 1)
+                std::sort(idx.begin() + at[s], idx.begin() + at[s + 1],
+                          [&](long a, long b) { return all[a].cluster < all[b].cluster; });
+        for (long k = 0; k < m; k++) rows[k] = all[idx[k]];
     }
     return 0;
 }

@@ -690,7 +690,7 @@ extern "C" int phx_rows(PhxCtx *c, PhxRow *rows, long cap, long *n) {
             for (long r = o; r < o + mi; r++) all[r].star += c->part_base[i];
             o += mi;
         }
-        const int ns = (int)c->st_h.size();
+        const int ns = c->part_base[c->nparts];   // the pass's batch (the loaded stars may have changed since)
         std::vector<long> at(ns + 1, 0);
         for (long r = 0; r < o; r++) {
             PHX_REQUIRE(all[r].star >= 0 && all[r].star < ns, "row star index out of the batch");

@@ -346,6 +346,30 @@ def test_long_tracks_wide_tile_masks(workload, oracle):
                 assert np.allclose(sub["chi2"], want["chi2"], rtol=1e-9, atol=1e-12)
 
 
+def test_scan_test_counts_match_the_cpu_order(workload):
+    """The GPU scan (level-2 entries collected across coarse tiles in rings) runs the tile tests of the CPU
+    model of the same two-level order (tests/native/cull_harness.cpp cc_scan2, fixture made by
+    tests/golden/make_scan_counts.py): the coarse level-1 tests exactly (they depend on the track lengths
+    only), the level-2 tests and the pairs with a tile left within 0.1% (the CPU runs on the oracle's tracks,
+    the GPU on its own FP64 tracks, equal to 1e-15, so a test at the edge may go the other way)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "scan_counts.json")) as f:
+        want = json.load(f)
+    cl = workload.make_clusters(want["clusters"], want["cluster_seed"])
+    st = workload.make_stars(want["stars"], want["star_seed"])
+    with hip.Engine(0) as e:
+        e.load_clusters(cl)
+        e.load_stars(st, 0)
+        e.screen(hip.FAST64, chi2_max=want["chi2_max"], p_min=0.0)
+        stats = e.pass_stats()
+    l2 = want["coarse_l1_passes"] + want["fine_tests"]
+    _report("scan_l2_tests_vs_cpu_rel", abs(stats["l2_tests"] - l2) / l2)
+    _report("scan_candidates_vs_cpu", stats["candidates"] - want["pairs_with_a_tile_left"])
+    assert stats["l1_tests"] == want["coarse_l1_tests"]
+    assert abs(stats["l2_tests"] - l2) <= 1e-3 * l2, (stats["l2_tests"], l2)
+    assert abs(stats["candidates"] - want["pairs_with_a_tile_left"]) <= 1e-3 * want["pairs_with_a_tile_left"]
+
+
 def test_split_passes_give_the_same_rows(workload):
     """A pass split into 2, 3 or 4 sub-passes on their own streams, with the last one's share of the stars from 30
     to 60%, gives the whole pass's rows, work counts and tracks (also with hipGraph replays)."""
